@@ -1,0 +1,413 @@
+"""Python mirror of aMat's Maps / aMatFree interface over the C ABI.
+
+Names, argument meanings and error behaviour follow the reference classes
+(include/maps.hpp, include/aMatFree.hpp, include/aMat.hpp of paralab/aMat) so
+that code written against them reads the same; every call goes to
+amat_amd/libamat_amd.so and raises AmatError with the par::Error name on a
+non-zero status.  Device arrays are torch tensors on a HIP device; host arrays
+are numpy arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+from typing import Sequence
+
+import numpy as np
+
+from . import _capi
+from ._capi import AmatError, call
+
+
+class BC_METH(enum.IntEnum):
+    """par::BC_METH (include/enums.hpp:20-22)."""
+    BC_IMATRIX = 0
+    BC_PENALTY = 1
+
+
+class MATFREE_TYPE(enum.IntEnum):
+    """par::MATFREE_TYPE (include/enums.hpp:56)."""
+    UNDEFINED = 0
+    QUASI_FREE = 1
+    FREE = 2
+    GPU_PURE = 3
+    GPU_OVER_CPU = 4
+    GPU_OVER_GPU = 5
+
+
+class Error(enum.IntEnum):
+    """par::Error (include/enums.hpp:24-37) plus this library's additions."""
+    SUCCESS = 0
+    INDEX_OUT_OF_BOUNDS = 1
+    UNKNOWN_ELEMENT_TYPE = 2
+    UNKNOWN_ELEMENT_STATUS = 3
+    NULL_L2G_MAP = 4
+    GHOST_NODE_NOT_FOUND = 5
+    UNKNOWN_MAT_TYPE = 6
+    WRONG_COMMUNICATION = 7
+    UNKNOWN_DOF_TYPE = 8
+    UNKNOWN_CONSTRAINT = 9
+    UNKNOWN_BC_METH = 10
+    NOT_IMPLEMENTED = 11
+    GLOBAL_DOF_ID_NOT_FOUND = 12
+    INVALID_ARGUMENT = 100
+    DEVICE_ERROR = 101
+    OUT_OF_MEMORY = 102
+    INVALID_STATE = 103
+    NOT_CONVERGED = 104
+    REJECTED = 105
+
+
+class PREC(enum.IntFlag):
+    FP64 = 1
+    FP32 = 2
+    BF16 = 4
+
+
+class SCATTER(enum.IntEnum):
+    DETERMINISTIC = 0
+    ATOMIC = 1
+
+
+class BCOP(enum.IntEnum):
+    NONE = 0
+    IDENTITY = 1       # aMatFree::MatMult_mf
+    PENALTY_DIAG = 2   # aMatBased::apply_bc_mat penalty branch
+
+
+def _ptr(a) -> int | None:
+    """Address of a contiguous numpy array or torch tensor (None for None)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        if not a.flags.c_contiguous:
+            raise ValueError("array must be C-contiguous")
+        return a.ctypes.data
+    if hasattr(a, "data_ptr"):
+        if not a.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return a.data_ptr()
+    raise TypeError(f"unsupported array type {type(a)}")
+
+
+def _arr(a, dtype) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=dtype))
+
+
+def _flatten_rows(rows, dtype) -> tuple[np.ndarray, np.ndarray]:
+    """A 2-D array or a sequence of 1-D rows -> (flat values, row lengths)."""
+    if isinstance(rows, np.ndarray) and rows.ndim == 2:
+        return _arr(rows.reshape(-1), dtype), np.full(rows.shape[0], rows.shape[1], dtype=np.uint32)
+    rows = [np.asarray(r) for r in rows]
+    lens = np.array([len(r) for r in rows], dtype=np.uint32)
+    flat = _arr(np.concatenate(rows) if rows else np.zeros(0), dtype)
+    return flat, lens
+
+
+def _stream_ptr(stream) -> int | None:
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return int(stream.cuda_stream)
+
+
+class Maps:
+    """par::Maps (include/maps.hpp) for one rank."""
+
+    def __init__(self, rank: int = 0, size: int = 1):
+        h = C.c_void_p()
+        call("amx_maps_create", rank, size, C.byref(h))
+        self._h = h
+        self.rank, self.size = rank, size
+        self._dpe = np.zeros(0, dtype=np.uint32)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _capi.lib().amx_maps_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- set-up --------------------------------------------------------
+    def set_map(self, n_elements_on_rank, element_to_rank_map, dofs_per_element, n_all_dofs_on_rank,
+                rank_to_global_map, owned_global_dof_range_begin, owned_global_dof_range_end, n_global_dofs,
+                rank_dof_counts=None):
+        """Maps::set_map (include/maps.hpp:398-451); element_to_rank_map is a
+        2-D array or a list of per-element rows."""
+        flat, lens = _flatten_rows(element_to_rank_map, np.uint32)
+        dpe = _arr(dofs_per_element, np.uint32) if dofs_per_element is not None else lens
+        if len(dpe) != n_elements_on_rank:
+            raise ValueError("dofs_per_element length differs from n_elements_on_rank")
+        r2g = _arr(rank_to_global_map, np.uint64)
+        counts = None if rank_dof_counts is None else _arr(rank_dof_counts, np.uint32)
+        call("amx_maps_set_map", self._h, n_elements_on_rank, _ptr(flat), _ptr(dpe), n_all_dofs_on_rank, _ptr(r2g),
+             int(owned_global_dof_range_begin), int(owned_global_dof_range_end), int(n_global_dofs), _ptr(counts))
+        self._dpe = dpe.copy()
+
+    def get_ghost_requests(self) -> tuple[np.ndarray, np.ndarray]:
+        """(counts per owner rank, ghost global ids) -- what buildScatterMap's
+        Alltoallv sends (include/maps.hpp:716-739)."""
+        info = self.info()
+        counts = np.zeros(self.size, dtype=np.uint32)
+        gids = np.zeros(max(info.n_recv, 1), dtype=np.uint64)
+        call("amx_maps_get_ghost_requests", self._h, _ptr(counts), _ptr(gids))
+        return counts, gids[:info.n_recv]
+
+    def set_peer_requests(self, counts, gids):
+        counts = _arr(counts, np.uint32)
+        gids = _arr(gids, np.uint64) if len(gids) else np.zeros(1, dtype=np.uint64)
+        call("amx_maps_set_peer_requests", self._h, _ptr(counts), _ptr(gids))
+
+    def set_bdr_map(self, constrainedDofs, prescribedValues, numConstraints=None):
+        """Maps::set_bdr_map (include/maps.hpp:828-880)."""
+        cd = _arr(constrainedDofs, np.uint64)
+        pv = _arr(prescribedValues, np.float64)
+        n = len(cd) if numConstraints is None else int(numConstraints)
+        if len(cd) == 0:
+            cd, pv = np.zeros(1, np.uint64), np.zeros(1)
+        call("amx_maps_set_bdr_map", self._h, _ptr(cd), _ptr(pv), n)
+
+    # -- getters (include/maps.hpp:85-276) ------------------------------
+    def info(self) -> _capi.MapsInfo:
+        inf = _capi.MapsInfo()
+        call("amx_maps_get_info", self._h, C.byref(inf))
+        return inf
+
+    def _get(self, fn: str, n: int, dtype):
+        out = np.zeros(max(n, 1), dtype=dtype)
+        call(fn, self._h, _ptr(out))
+        return out[:n]
+
+    def get_NumElems(self): return self.info().n_elems
+    def get_NumDofs(self): return self.info().n_owned
+    def get_NumDofsTotal(self): return self.info().n_total
+    def get_NumPreGhostDofs(self): return self.info().n_pre_ghost
+    def get_NumPostGhostDofs(self): return self.info().n_post_ghost
+    def get_DofsPerElem(self): return self._dpe.copy()
+    def get_LocalMap(self): return self._get("amx_maps_get_local_map", self.info().n_entries, np.uint32)
+    def get_Map(self): return self._get("amx_maps_get_global_map", self.info().n_entries, np.uint64)
+    def get_Local2Global(self): return self._get("amx_maps_get_local2global", self.info().n_total, np.uint64)
+    def get_localID2Rank(self): return self._get("amx_maps_get_localid2rank", self.info().n_total, np.uint32)
+    def get_ownedFreeDofs(self): return self._get("amx_maps_get_owned_free", self.info().n_owned_free, np.uint64)
+    def get_LocalDofScan(self): return self._get("amx_maps_get_dof_scan", self.size, np.uint64)
+
+    def get_BdrMap(self):
+        n = self.info().n_entries
+        m, v = np.zeros(max(n, 1), np.uint32), np.zeros(max(n, 1))
+        call("amx_maps_get_bdr_map", self._h, _ptr(m), _ptr(v))
+        return m[:n]
+
+    def get_PresValMap(self):
+        n = self.info().n_entries
+        m, v = np.zeros(max(n, 1), np.uint32), np.zeros(max(n, 1))
+        call("amx_maps_get_bdr_map", self._h, _ptr(m), _ptr(v))
+        return v[:n]
+
+    def _constraints(self):
+        n = self.info().n_owned_constraints
+        d, v = np.zeros(max(n, 1), np.uint64), np.zeros(max(n, 1))
+        call("amx_maps_get_owned_constraints", self._h, _ptr(d), _ptr(v))
+        return d[:n], v[:n]
+
+    def get_ownedConstrainedDofs(self): return self._constraints()[0]
+    def get_ownedPrescribedValues(self): return self._constraints()[1]
+    def get_n_owned_constraints(self): return self.info().n_owned_constraints
+
+    def _elem_lists(self):
+        inf = self.info()
+        a, b = np.zeros(max(inf.n_independent, 1), np.uint32), np.zeros(max(inf.n_dependent, 1), np.uint32)
+        call("amx_maps_get_element_lists", self._h, _ptr(a), _ptr(b))
+        return a[:inf.n_independent], b[:inf.n_dependent]
+
+    def get_independentElem(self): return self._elem_lists()[0]
+    def get_dependentElem(self): return self._elem_lists()[1]
+
+    def _send(self):
+        inf = self.info()
+        c, o = np.zeros(self.size, np.uint32), np.zeros(self.size, np.uint32)
+        ids = np.zeros(max(inf.n_send, 1), np.uint32)
+        call("amx_maps_get_send_lists", self._h, _ptr(c), _ptr(o), _ptr(ids))
+        return c, o, ids[:inf.n_send]
+
+    def _recv(self):
+        c, o = np.zeros(self.size, np.uint32), np.zeros(self.size, np.uint32)
+        call("amx_maps_get_recv_lists", self._h, _ptr(c), _ptr(o))
+        return c, o
+
+    def get_SendDofCounts(self): return self._send()[0]
+    def get_SendDofOffset(self): return self._send()[1]
+    def get_SendDofIds(self): return self._send()[2]
+    def get_SendRankIds(self): return np.nonzero(self._send()[0])[0].astype(np.uint32)
+    def get_RecvDofCounts(self): return self._recv()[0]
+    def get_RecvDofOffset(self): return self._recv()[1]
+    def get_RecvRankIds(self): return np.nonzero(self._recv()[0])[0].astype(np.uint32)
+
+
+class aMatFree:
+    """par::aMatFree (include/aMatFree.hpp) running MATFREE_TYPE 3/4/5 on one GPU."""
+
+    def __init__(self, mesh_maps: Maps, bcType: BC_METH = BC_METH.BC_IMATRIX, device: int = 0):
+        h = C.c_void_p()
+        call("amx_op_create", mesh_maps.handle, int(bcType), int(device), C.byref(h))
+        self._h = h
+        self.maps = mesh_maps
+        self.device = device
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _capi.lib().amx_op_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- set-up --------------------------------------------------------
+    def set_matfree_type(self, t: MATFREE_TYPE):
+        call("amx_op_set_matfree_type", self._h, int(t))
+
+    def set_num_streams(self, n: int):
+        call("amx_op_set_num_streams", self._h, int(n))
+
+    def set_element_matrix(self, eid: int, e_mat, block_i: int = 0, block_j: int = 0, blocks_dim: int = 1):
+        """aMatFree::set_element_matrix (include/aMatFree.hpp:854-880); e_mat is an
+        n_b x n_b array (row-major numpy)."""
+        m = _arr(e_mat, np.float64)
+        call("amx_op_set_element_matrix", self._h, eid, _ptr(m), 0, block_i, block_j, blocks_dim)
+
+    def set_element_matrices(self, first_eid: int, e_mats):
+        m = _arr(e_mats, np.float64)
+        count = m.shape[0] if m.ndim == 3 else None
+        if count is None:
+            raise ValueError("e_mats must be (count, n, n)")
+        call("amx_op_set_element_matrices", self._h, first_eid, count, _ptr(m), 0)
+
+    def set_element_matrices_scaled(self, k_ref, scale):
+        kr = _arr(k_ref, np.float64)
+        sc = _arr(scale, np.float64)
+        call("amx_op_set_element_matrices_scaled", self._h, _ptr(kr), 0, _ptr(sc))
+
+    def set_options(self, precision=PREC.FP64, scatter=SCATTER.DETERMINISTIC):
+        call("amx_op_set_options", self._h, int(precision), int(scatter))
+
+    def set_penalty(self, p: float):
+        call("amx_op_set_penalty", self._h, float(p))
+
+    def set_bc_operator(self, mode: BCOP):
+        call("amx_op_set_bc_operator", self._h, int(mode))
+
+    def finalize(self):
+        call("amx_op_finalize", self._h)
+
+    def info(self) -> _capi.OpInfo:
+        inf = _capi.OpInfo()
+        call("amx_op_get_info", self._h, C.byref(inf))
+        return inf
+
+    # -- host-array forms ----------------------------------------------
+    def matvec(self, v: np.ndarray, u: np.ndarray, isGhosted: bool = False):
+        """aMatFree::matvec (include/aMatFree.hpp:1681-1775): v = K u."""
+        uu = _arr(u, np.float64)
+        if v.dtype != np.float64 or not v.flags.c_contiguous:
+            raise ValueError("v must be a contiguous float64 array")
+        call("amx_op_matvec", self._h, _ptr(v), _ptr(uu), int(bool(isGhosted)))
+        return v
+
+    def MatMult_mf(self, u: np.ndarray) -> np.ndarray:
+        """aMatFree::MatMult_mf (include/aMatFree.hpp:3893-3941) on owned-length arrays."""
+        uu = _arr(u, np.float64)
+        vv = np.zeros_like(uu)
+        call("amx_op_matmult_mf", self._h, _ptr(vv), _ptr(uu))
+        return vv
+
+    def get_diagonal(self) -> np.ndarray:
+        """MatGetDiagonal_mf (include/aMatFree.hpp:3944-3974), owned length."""
+        d = np.zeros(max(self.info().n_owned, 1))
+        call("amx_op_get_diagonal", self._h, _ptr(d))
+        return d[:self.info().n_owned]
+
+    def apply_bc(self, rhs: np.ndarray) -> np.ndarray:
+        """aMatFree::apply_bc -> apply_bc_rhs (include/aMatFree.hpp:957-971, 4536-4700)."""
+        r = _arr(rhs, np.float64).copy()
+        call("amx_op_apply_bc_rhs", self._h, _ptr(r))
+        return r
+
+    # -- device forms (torch tensors) -----------------------------------
+    def matvec_dev(self, v, u, stream=None):
+        call("amx_op_matvec_dev", self._h, _ptr(v), _ptr(u), _stream_ptr(stream))
+
+    def matmult_dev(self, v, u, stream=None):
+        call("amx_op_matmult_dev", self._h, _ptr(v), _ptr(u), _stream_ptr(stream))
+
+    def matmult_f32_dev(self, precision: PREC, v, u, stream=None):
+        call("amx_op_matmult_f32_dev", self._h, int(precision), _ptr(v), _ptr(u), _stream_ptr(stream))
+
+    def matvec_begin_dev(self, v, u, apply_bc=True, stream=None):
+        call("amx_op_matvec_begin_dev", self._h, _ptr(v), _ptr(u), int(apply_bc), _stream_ptr(stream))
+
+    def matvec_end_dev(self, v, u, apply_bc=True, stream=None):
+        call("amx_op_matvec_end_dev", self._h, _ptr(v), _ptr(u), int(apply_bc), _stream_ptr(stream))
+
+    def halo_import_dev(self, u, halo, stream=None):
+        call("amx_op_halo_import_dev", self._h, _ptr(u), _ptr(halo), _stream_ptr(stream))
+
+    def halo_export_dev(self, v, halo, stream=None):
+        call("amx_op_halo_export_dev", self._h, _ptr(v), _ptr(halo), _stream_ptr(stream))
+
+    def halo_accumulate_dev(self, v, recv, stream=None):
+        call("amx_op_halo_accumulate_dev", self._h, _ptr(v), _ptr(recv), _stream_ptr(stream))
+
+    def bc_fixup_dev(self, v, u, stream=None):
+        call("amx_op_bc_fixup_dev", self._h, _ptr(v), _ptr(u), _stream_ptr(stream))
+
+    def get_diagonal_dev(self, diag, stream=None):
+        call("amx_op_get_diagonal_dev", self._h, _ptr(diag), _stream_ptr(stream))
+
+    def time_kernels_dev(self, n_iter, v, u, stream=None):
+        """Mean ms of the element kernels, the gather and the whole application
+        over n_iter runs (hipEvents on the launch stream)."""
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        call("amx_op_time_kernels_dev", self._h, int(n_iter), _ptr(v), _ptr(u), C.byref(a), C.byref(b), C.byref(c),
+             _stream_ptr(stream))
+        return a.value, b.value, c.value
+
+    def stream(self) -> int:
+        return _capi.lib().amx_op_stream(self._h) or 0
+
+    def synchronize(self):
+        call("amx_op_synchronize", self._h)
+
+    # -- solvers ---------------------------------------------------------
+    def cg(self, b, x, rtol=1e-10, max_it=10000, stream=None) -> _capi.SolveStats:
+        st = _capi.SolveStats()
+        call("amx_cg_solve_dev", self._h, _ptr(b), _ptr(x), float(rtol), int(max_it), C.byref(st),
+             _stream_ptr(stream))
+        return st
+
+    def ir_cg(self, precision: PREC, b, x, rtol=1e-10, inner_rtol=1e-2, max_outer=50, max_inner=10000,
+              stream=None) -> _capi.SolveStats:
+        st = _capi.SolveStats()
+        call("amx_ir_cg_solve_dev", self._h, int(precision), _ptr(b), _ptr(x), float(rtol), float(inner_rtol),
+             int(max_outer), int(max_inner), C.byref(st), _stream_ptr(stream))
+        return st
+
+
+def device_count() -> int:
+    return int(_capi.lib().amx_device_count())
+
+
+def version() -> str:
+    return _capi.lib().amx_version().decode()
+
+
+__all__ = ["AmatError", "BC_METH", "BCOP", "Error", "MATFREE_TYPE", "Maps", "PREC", "SCATTER", "aMatFree",
+           "device_count", "version"]
+
+
+def flatten_rows(rows: Sequence) -> tuple[np.ndarray, np.ndarray]:
+    return _flatten_rows(rows, np.uint32)

@@ -1,0 +1,582 @@
+// amx_kernels.hip -- gfx950 kernels of the assembly-free EbE operator.
+//
+// Hot path (per matvec):
+//   ebe_tile_kernel  one 64-lane wave per tile of G = 64/n elements: lane
+//                    (g, r) gathers u[map] (coalesced index read, cached u
+//                    read), streams its row of K_e as P-column packs with
+//                    non-temporal 16-byte loads, and forms ve = K_e u_e with
+//                    the column values broadcast across the wave (readlane for
+//                    G = 1, ds_bpermute otherwise).  K has no reuse, so it is
+//                    never staged in LDS.
+//   gather_kernel    v[d] = sum of the element rows that touch d, in the
+//                    reference's element order (node -> (element, row)
+//                    transposed lists), with the BC rows fused.
+// Blocks are remapped so that each of the 8 XCDs walks a contiguous range of
+// tiles / DoFs: its L2 then holds one slab of u / ve instead of all of it.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "amx_internal.hpp"
+#include "amx_kernels.hpp"
+
+namespace amx {
+
+namespace {
+
+constexpr uint32_t kFlagBit = 0x80000000u;
+constexpr uint32_t kIdxMask = 0x7fffffffu;
+constexpr uint32_t kPadSlot = 0xffffffffu;
+
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8).  Return a logical block index such that XCD x gets the contiguous
+// logical range [x*q + min(x, r), ... + q + (x < r)), q = nb / 8, r = nb % 8.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
+{
+    const uint32_t x = b & 7u, i = b >> 3;
+    const uint32_t q = nb >> 3, r = nb & 7u;
+    return x * q + (x < r ? x : r) + i;
+}
+
+template <typename KT> struct KSize;
+template <> struct KSize<double> { static constexpr int v = 8; };
+template <> struct KSize<float> { static constexpr int v = 4; };
+template <> struct KSize<uint16_t> { static constexpr int v = 2; };
+
+// columns per lane load: the largest P in {8,4,2,1} dividing n with
+// P * sizeof(KT) <= 16 bytes
+__host__ __device__ constexpr int pack_of(int n, int ksize)
+{
+    return (n % 8 == 0 && 8 * ksize <= 16) ? 8
+           : (n % 4 == 0 && 4 * ksize <= 16) ? 4
+           : (n % 2 == 0 && 2 * ksize <= 16) ? 2
+                                              : 1;
+}
+
+__device__ __forceinline__ float bf16_to_f32(uint32_t bits16) { return __uint_as_float(bits16 << 16); }
+
+__device__ __forceinline__ uint16_t f64_to_bf16(double x)
+{
+    // round to FP32 then to BF16, both to nearest even
+    const uint32_t b = __float_as_uint((float)x);
+    const uint32_t bias = 0x7fffu + ((b >> 16) & 1u);
+    return (uint16_t)((b + bias) >> 16);
+}
+
+template <typename KT> __device__ __forceinline__ KT from_f64(double x);
+template <> __device__ __forceinline__ double from_f64<double>(double x) { return x; }
+template <> __device__ __forceinline__ float from_f64<float>(double x) { return (float)x; }
+template <> __device__ __forceinline__ uint16_t from_f64<uint16_t>(double x) { return f64_to_bf16(x); }
+
+// P consecutive K values of one lane, read with one non-temporal load.
+template <typename KT, int P> struct KPack;
+
+template <int P> struct KPack<double, P> {
+    typedef double vec __attribute__((ext_vector_type(P)));
+    vec v;
+    __device__ __forceinline__ void load(const double* p) { v = __builtin_nontemporal_load((const vec*)p); }
+    __device__ __forceinline__ double get(int i) const { return v[i]; }
+};
+template <> struct KPack<double, 1> {
+    double v;
+    __device__ __forceinline__ void load(const double* p) { v = __builtin_nontemporal_load(p); }
+    __device__ __forceinline__ double get(int) const { return v; }
+};
+template <int P> struct KPack<float, P> {
+    typedef float vec __attribute__((ext_vector_type(P)));
+    vec v;
+    __device__ __forceinline__ void load(const float* p) { v = __builtin_nontemporal_load((const vec*)p); }
+    __device__ __forceinline__ float get(int i) const { return v[i]; }
+};
+template <> struct KPack<float, 1> {
+    float v;
+    __device__ __forceinline__ void load(const float* p) { v = __builtin_nontemporal_load(p); }
+    __device__ __forceinline__ float get(int) const { return v; }
+};
+// BF16: P values packed two per 32-bit word
+template <int P> struct KPack<uint16_t, P> {
+    static constexpr int W = P / 2;
+    typedef uint32_t vec __attribute__((ext_vector_type(W)));
+    vec v;
+    __device__ __forceinline__ void load(const uint16_t* p) { v = __builtin_nontemporal_load((const vec*)p); }
+    __device__ __forceinline__ float get(int i) const
+    {
+        const uint32_t w = v[i >> 1];
+        return bf16_to_f32((i & 1) ? (w >> 16) : (w & 0xffffu));
+    }
+};
+template <> struct KPack<uint16_t, 2> {
+    uint32_t v;
+    __device__ __forceinline__ void load(const uint16_t* p) { v = __builtin_nontemporal_load((const uint32_t*)p); }
+    __device__ __forceinline__ float get(int i) const { return bf16_to_f32(i ? (v >> 16) : (v & 0xffffu)); }
+};
+template <> struct KPack<uint16_t, 1> {
+    uint16_t v;
+    __device__ __forceinline__ void load(const uint16_t* p) { v = __builtin_nontemporal_load(p); }
+    __device__ __forceinline__ float get(int) const { return bf16_to_f32(v); }
+};
+
+// acc + k*u.  FP64: product and sum rounded separately (no contraction), the
+// arithmetic of the reference's scalar GEMV (include/aMatFree.hpp:1994-1999,
+// built without FMA), so the deterministic FP64 path reproduces it bit for bit.
+__device__ __forceinline__ double madd(double acc, double k, double u)
+{
+#pragma clang fp contract(off)
+    const double p = k * u;
+    return acc + p;
+}
+__device__ __forceinline__ float madd(float acc, float k, float u) { return __builtin_fmaf(k, u, acc); }
+
+// value of lane `src` (uniform, compile-time after unrolling) to all lanes
+__device__ __forceinline__ double readlane(double x, int src)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), src);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float readlane(float x, int src)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), src));
+}
+
+template <int G, int N, typename AT> __device__ __forceinline__ AT bcast(AT x, uint32_t g, int c)
+{
+    if constexpr (G == 1) {
+        return readlane(x, c);
+    } else {
+        return __shfl(x, (int)(g * N + c), 64);
+    }
+}
+
+template <typename VT> __device__ __forceinline__ void atomic_add(VT* p, VT x) { unsafeAtomicAdd(p, x); }
+
+// ---------------------------------------------------------------- EbE tile
+template <int N, typename KT, typename VT, bool MASKED, bool ATOMIC>
+__global__ __launch_bounds__(256) void ebe_tile_kernel(const KT* __restrict__ K, const uint32_t* __restrict__ map,
+                                                      const VT* __restrict__ u, VT* __restrict__ out,
+                                                      uint64_t ve_base, uint32_t tile_begin, uint32_t n_tiles,
+                                                      const uint32_t* __restrict__ skip)
+{
+    using AT = VT;
+    if (skip && *skip) return;  // converged solver: nothing to do
+    constexpr int G = 64 / N;
+    constexpr int GN = G * N;
+    constexpr int P = pack_of(N, KSize<KT>::v);
+    constexpr int NP = N / P;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t tloc = xcd_block(blockIdx.x, gridDim.x) * 4u + (threadIdx.x >> 6);
+    if (tloc >= n_tiles) return;  // uniform per wave
+    const uint32_t t = tile_begin + tloc;
+    // lanes past G*N mirror the last active lane (same addresses, no traffic)
+    const uint32_t lc = lane < (uint32_t)GN ? lane : (uint32_t)GN - 1u;
+    const uint32_t g = lc / N;
+    const uint64_t row = (uint64_t)t * GN + lc;  // = slot * N + r
+    const uint32_t m = map[row];
+    VT uv = u[m & kIdxMask];
+    if (MASKED && (m & kFlagBit)) uv = VT(0);
+    const AT ue = uv;
+    const KT* kp = K + ((uint64_t)t * NP * GN + lc) * P;
+    AT acc = AT(0);
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+        KPack<KT, P> k;
+        k.load(kp + (uint64_t)j * GN * P);
+#pragma unroll
+        for (int p = 0; p < P; ++p) acc = madd(acc, (AT)k.get(p), bcast<G, N>(ue, g, j * P + p));
+    }
+    if (lane < (uint32_t)GN) {
+        if constexpr (ATOMIC) atomic_add(&out[m & kIdxMask], acc);
+        else out[ve_base + row] = acc;
+    }
+}
+
+// Any n: one wave per element, column-major storage (G = 1, P = 1), element
+// vector staged in LDS, rows strided over the lanes.
+constexpr int kGenericMaxN = 1024;
+
+template <typename KT, typename VT, bool MASKED, bool ATOMIC>
+__global__ __launch_bounds__(256) void ebe_generic_kernel(const KT* __restrict__ K, const uint32_t* __restrict__ map,
+                                                         const VT* __restrict__ u, VT* __restrict__ out,
+                                                         uint64_t ve_base, uint32_t n, uint32_t slot_begin,
+                                                         uint32_t n_slots, const uint32_t* __restrict__ skip)
+{
+    if (skip && *skip) return;
+    __shared__ VT ue[4][kGenericMaxN];
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t sloc = xcd_block(blockIdx.x, gridDim.x) * 4u + w;
+    if (sloc >= n_slots) return;
+    const uint64_t slot = slot_begin + sloc;
+    for (uint32_t c = lane; c < n; c += 64) {
+        const uint32_t m = map[slot * n + c];
+        VT uv = u[m & kIdxMask];
+        if (MASKED && (m & kFlagBit)) uv = VT(0);
+        ue[w][c] = uv;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    const KT* ke = K + slot * n * n;
+    for (uint32_t r = lane; r < n; r += 64) {
+        VT acc = VT(0);
+        for (uint32_t c = 0; c < n; c++) {
+            KPack<KT, 1> k;
+            k.load(ke + (uint64_t)c * n + r);
+            acc = madd(acc, (VT)k.get(0), ue[w][c]);
+        }
+        if constexpr (ATOMIC) atomic_add(&out[map[slot * n + r] & kIdxMask], acc);
+        else out[ve_base + slot * n + r] = acc;
+    }
+}
+
+// ------------------------------------------------------------------ gather
+template <typename VT, int BC>
+__global__ __launch_bounds__(256) void gather_kernel(const uint32_t* __restrict__ off, const uint32_t* __restrict__ pos,
+                                                    const VT* __restrict__ ve, const VT* __restrict__ u,
+                                                    const uint8_t* __restrict__ flag, const double* __restrict__ coef,
+                                                    VT* __restrict__ v, uint32_t n, const uint32_t* __restrict__ skip)
+{
+    if (skip && *skip) return;
+    const uint32_t d = xcd_block(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
+    if (d >= n) return;
+    const uint32_t b = off[d], e = off[d + 1];
+    VT s = VT(0);
+    for (uint32_t q = b; q < e; ++q) s += ve[pos[q]];
+    if constexpr (BC == 1) {
+        if (flag[d]) s = u[d];
+    } else if constexpr (BC == 2) {
+        if (flag[d]) s = (VT)((double)s + coef[d] * (double)u[d]);  // penalty row in FP64
+    }
+    v[d] = s;
+}
+
+template <typename VT, int BC>
+__global__ void bc_fixup_kernel(const uint32_t* __restrict__ cdofs, uint32_t nc, const double* __restrict__ coef,
+                                const VT* __restrict__ u, VT* __restrict__ v)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    const uint32_t d = cdofs[i];
+    if constexpr (BC == 1) v[d] = u[d];
+    else v[d] = (VT)((double)v[d] + coef[d] * (double)u[d]);
+}
+
+// ------------------------------------------------------------------ set-up
+struct TiledIndex {
+    uint32_t N, G, P;
+    // destination index -> (slot, r, c)
+    __device__ __forceinline__ void decompose(uint64_t idx, uint64_t& slot, uint32_t& r, uint32_t& c) const
+    {
+        const uint32_t p = (uint32_t)(idx % P);
+        uint64_t rest = idx / P;
+        const uint32_t GN = G * N;
+        const uint32_t l = (uint32_t)(rest % GN);
+        rest /= GN;
+        const uint32_t NP = N / P;
+        const uint32_t j = (uint32_t)(rest % NP);
+        const uint64_t t = rest / NP;
+        const uint32_t g = l / N;
+        r = l - g * N;
+        c = j * P + p;
+        slot = t * G + g;
+    }
+};
+
+template <typename KT>
+__global__ void relayout_rowmajor_kernel(const double* __restrict__ src, const uint32_t* __restrict__ slot_elem,
+                                         uint64_t total, TiledIndex ti, KT* __restrict__ dst)
+{
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    uint64_t slot;
+    uint32_t r, c;
+    ti.decompose(idx, slot, r, c);
+    const uint32_t e = slot_elem[slot];
+    const double val = (e == kPadSlot) ? 0.0 : src[(uint64_t)e * ti.N * ti.N + (uint64_t)r * ti.N + c];
+    dst[idx] = from_f64<KT>(val);
+}
+
+template <typename KT>
+__global__ void relayout_scaled_kernel(const double* __restrict__ kref, const double* __restrict__ scale,
+                                       const uint32_t* __restrict__ slot_elem, uint64_t total, TiledIndex ti,
+                                       KT* __restrict__ dst)
+{
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    uint64_t slot;
+    uint32_t r, c;
+    ti.decompose(idx, slot, r, c);
+    const uint32_t e = slot_elem[slot];
+    const double val = (e == kPadSlot) ? 0.0 : scale[e] * kref[(uint64_t)r * ti.N + c];
+    dst[idx] = from_f64<KT>(val);
+}
+
+__global__ void extract_diag_kernel(const double* __restrict__ K, TiledIndex ti, uint64_t n_rows, double* __restrict__ ve)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;  // slot * N + r
+    if (i >= n_rows) return;
+    const uint64_t slot = i / ti.N;
+    const uint32_t r = (uint32_t)(i - slot * ti.N);
+    const uint64_t t = slot / ti.G;
+    const uint32_t g = (uint32_t)(slot - t * ti.G);
+    const uint32_t GN = ti.G * ti.N, NP = ti.N / ti.P;
+    const uint64_t k = ((t * NP + r / ti.P) * GN + g * ti.N + r) * ti.P + r % ti.P;
+    ve[i] = K[k];
+}
+
+template <typename VT> __global__ void fill_kernel(VT* __restrict__ p, uint64_t n, VT value)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = value;
+}
+
+__global__ void coef_kernel(const double* __restrict__ diag, const uint32_t* __restrict__ cdofs, uint32_t nc,
+                            double penalty, double* __restrict__ c64)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc) return;
+    const uint32_t d = cdofs[i];
+    c64[d] = penalty - diag[d];
+}
+
+__global__ void set_entries_kernel(double* __restrict__ u, const uint32_t* __restrict__ ids, uint32_t n, double value)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) u[ids[i]] = value;
+}
+
+__global__ void halo_accumulate_kernel(double* __restrict__ v, const double* __restrict__ recv,
+                                       const uint32_t* __restrict__ dst, const uint32_t* __restrict__ off,
+                                       const uint32_t* __restrict__ src, uint32_t n_unique)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_unique) return;
+    double s = v[dst[i]];
+    for (uint32_t q = off[i]; q < off[i + 1]; ++q) s += recv[src[q]];
+    v[dst[i]] = s;
+}
+
+__global__ void d2f_kernel(const double* __restrict__ a, float* __restrict__ b, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+__global__ void f2d_kernel(const float* __restrict__ a, double* __restrict__ b, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
+}
+
+inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
+
+template <int N, typename KT, typename VT>
+void launch_tile(bool masked, bool atomic, const EbeArgs& a, hipStream_t s)
+{
+    const dim3 grid(blocks_for(a.n_tiles, 4)), block(256);
+    const KT* K = static_cast<const KT*>(a.K);
+    const VT* u = static_cast<const VT*>(a.u);
+    VT* out = static_cast<VT*>(a.out);
+    if (masked) {
+        if (atomic) ebe_tile_kernel<N, KT, VT, true, true><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.tile_begin, a.n_tiles, a.skip);
+        else ebe_tile_kernel<N, KT, VT, true, false><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.tile_begin, a.n_tiles, a.skip);
+    } else {
+        if (atomic) ebe_tile_kernel<N, KT, VT, false, true><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.tile_begin, a.n_tiles, a.skip);
+        else ebe_tile_kernel<N, KT, VT, false, false><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.tile_begin, a.n_tiles, a.skip);
+    }
+}
+
+template <typename KT, typename VT>
+void launch_generic(uint32_t n, bool masked, bool atomic, const EbeArgs& a, hipStream_t s)
+{
+    const dim3 grid(blocks_for(a.n_tiles, 4)), block(256);
+    const KT* K = static_cast<const KT*>(a.K);
+    const VT* u = static_cast<const VT*>(a.u);
+    VT* out = static_cast<VT*>(a.out);
+    if (masked) {
+        if (atomic) ebe_generic_kernel<KT, VT, true, true><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, n, a.tile_begin, a.n_tiles, a.skip);
+        else ebe_generic_kernel<KT, VT, true, false><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, n, a.tile_begin, a.n_tiles, a.skip);
+    } else {
+        if (atomic) ebe_generic_kernel<KT, VT, false, true><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, n, a.tile_begin, a.n_tiles, a.skip);
+        else ebe_generic_kernel<KT, VT, false, false><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, n, a.tile_begin, a.n_tiles, a.skip);
+    }
+}
+
+template <typename KT, typename VT>
+void dispatch_n(const Layout& L, bool masked, bool atomic, const EbeArgs& a, hipStream_t s)
+{
+    if (!L.specialized) return launch_generic<KT, VT>(L.n, masked, atomic, a, s);
+    switch (L.n) {
+    case 4: return launch_tile<4, KT, VT>(masked, atomic, a, s);
+    case 8: return launch_tile<8, KT, VT>(masked, atomic, a, s);
+    case 10: return launch_tile<10, KT, VT>(masked, atomic, a, s);
+    case 12: return launch_tile<12, KT, VT>(masked, atomic, a, s);
+    case 20: return launch_tile<20, KT, VT>(masked, atomic, a, s);
+    case 24: return launch_tile<24, KT, VT>(masked, atomic, a, s);
+    case 30: return launch_tile<30, KT, VT>(masked, atomic, a, s);
+    case 60: return launch_tile<60, KT, VT>(masked, atomic, a, s);
+    default: fail(AMX_INVALID_STATE, "no tile kernel for n = " + std::to_string(L.n));
+    }
+}
+
+bool has_tile_kernel(uint32_t n)
+{
+    switch (n) {
+    case 4: case 8: case 10: case 12: case 20: case 24: case 30: case 60: return true;
+    default: return false;
+    }
+}
+
+}  // namespace
+
+size_t ktype_size(KType kt) { return kt == KType::F64 ? 8 : kt == KType::F32 ? 4 : 2; }
+
+Layout choose_layout(uint32_t n, KType kt)
+{
+    Layout L;
+    L.n = n;
+    if (has_tile_kernel(n)) {
+        L.specialized = true;
+        L.G = 64 / n;
+        L.P = (uint32_t)pack_of((int)n, (int)ktype_size(kt));
+    } else {
+        AMX_REQUIRE(n <= (uint32_t)kGenericMaxN, AMX_NOT_IMPLEMENTED,
+                    "elements with more than " + std::to_string(kGenericMaxN) + " DoFs");
+        L.specialized = false;
+        L.G = 1;
+        L.P = 1;
+    }
+    return L;
+}
+
+void launch_relayout_from_rowmajor(const double* src, const uint32_t* slot_elem, uint32_t n_slots, const Layout& L,
+                                   KType kt, void* dst, hipStream_t s)
+{
+    const uint64_t total = (uint64_t)n_slots * L.n * L.n;
+    if (total == 0) return;
+    const TiledIndex ti{L.n, L.G, L.P};
+    const uint32_t nb = blocks_for(total, 256);
+    switch (kt) {
+    case KType::F64: relayout_rowmajor_kernel<double><<<nb, 256, 0, s>>>(src, slot_elem, total, ti, (double*)dst); break;
+    case KType::F32: relayout_rowmajor_kernel<float><<<nb, 256, 0, s>>>(src, slot_elem, total, ti, (float*)dst); break;
+    case KType::BF16: relayout_rowmajor_kernel<uint16_t><<<nb, 256, 0, s>>>(src, slot_elem, total, ti, (uint16_t*)dst); break;
+    }
+    AMX_HIP(hipGetLastError());
+}
+
+void launch_relayout_scaled(const double* k_ref, const double* scale, const uint32_t* slot_elem, uint32_t n_slots,
+                            const Layout& L, KType kt, void* dst, hipStream_t s)
+{
+    const uint64_t total = (uint64_t)n_slots * L.n * L.n;
+    if (total == 0) return;
+    const TiledIndex ti{L.n, L.G, L.P};
+    const uint32_t nb = blocks_for(total, 256);
+    switch (kt) {
+    case KType::F64: relayout_scaled_kernel<double><<<nb, 256, 0, s>>>(k_ref, scale, slot_elem, total, ti, (double*)dst); break;
+    case KType::F32: relayout_scaled_kernel<float><<<nb, 256, 0, s>>>(k_ref, scale, slot_elem, total, ti, (float*)dst); break;
+    case KType::BF16: relayout_scaled_kernel<uint16_t><<<nb, 256, 0, s>>>(k_ref, scale, slot_elem, total, ti, (uint16_t*)dst); break;
+    }
+    AMX_HIP(hipGetLastError());
+}
+
+void launch_extract_diag(const double* K, const Layout& L, uint32_t n_slots, double* ve, hipStream_t s)
+{
+    const uint64_t rows = (uint64_t)n_slots * L.n;
+    if (rows == 0) return;
+    extract_diag_kernel<<<blocks_for(rows, 256), 256, 0, s>>>(K, TiledIndex{L.n, L.G, L.P}, rows, ve);
+    AMX_HIP(hipGetLastError());
+}
+
+void launch_ebe(const Layout& L, KType kt, bool vt_double, bool masked, bool atomic, const EbeArgs& a, hipStream_t s)
+{
+    if (a.n_tiles == 0) return;
+    if (vt_double) {
+        AMX_REQUIRE(kt == KType::F64, AMX_INVALID_ARGUMENT, "FP64 vectors need FP64 element matrices");
+        dispatch_n<double, double>(L, masked, atomic, a, s);
+    } else if (kt == KType::F32) {
+        dispatch_n<float, float>(L, masked, atomic, a, s);
+    } else if (kt == KType::BF16) {
+        dispatch_n<uint16_t, float>(L, masked, atomic, a, s);
+    } else {
+        fail(AMX_INVALID_ARGUMENT, "FP32 vectors need FP32 or BF16 element matrices");
+    }
+    AMX_HIP(hipGetLastError());
+}
+
+void launch_gather(bool vt_double, int bc, const uint32_t* off, const uint32_t* pos, const void* ve, const void* u,
+                   const uint8_t* flag, const double* coef, void* v, uint32_t n, hipStream_t s, const uint32_t* skip)
+{
+    if (n == 0) return;
+    const uint32_t nb = blocks_for(n, 256);
+#define AMX_GATHER(VT, BC)                                                                                         \
+    gather_kernel<VT, BC><<<nb, 256, 0, s>>>(off, pos, (const VT*)ve, (const VT*)u, flag, coef, (VT*)v, n, skip)
+    if (vt_double) {
+        if (bc == 0) AMX_GATHER(double, 0);
+        else if (bc == 1) AMX_GATHER(double, 1);
+        else AMX_GATHER(double, 2);
+    } else {
+        if (bc == 0) AMX_GATHER(float, 0);
+        else if (bc == 1) AMX_GATHER(float, 1);
+        else AMX_GATHER(float, 2);
+    }
+#undef AMX_GATHER
+    AMX_HIP(hipGetLastError());
+}
+
+void launch_bc_fixup(bool vt_double, int bc, const uint32_t* cdofs, uint32_t nc, const double* coef, const void* u,
+                     void* v, hipStream_t s)
+{
+    if (nc == 0 || bc == 0) return;
+    const uint32_t nb = blocks_for(nc, 256);
+    if (vt_double) {
+        if (bc == 1) bc_fixup_kernel<double, 1><<<nb, 256, 0, s>>>(cdofs, nc, coef, (const double*)u, (double*)v);
+        else bc_fixup_kernel<double, 2><<<nb, 256, 0, s>>>(cdofs, nc, coef, (const double*)u, (double*)v);
+    } else {
+        if (bc == 1) bc_fixup_kernel<float, 1><<<nb, 256, 0, s>>>(cdofs, nc, coef, (const float*)u, (float*)v);
+        else bc_fixup_kernel<float, 2><<<nb, 256, 0, s>>>(cdofs, nc, coef, (const float*)u, (float*)v);
+    }
+    AMX_HIP(hipGetLastError());
+}
+
+void launch_fill(void* p, uint64_t n, bool vt_double, double value, hipStream_t s)
+{
+    if (n == 0) return;
+    if (vt_double) fill_kernel<double><<<blocks_for(n, 256), 256, 0, s>>>((double*)p, n, value);
+    else fill_kernel<float><<<blocks_for(n, 256), 256, 0, s>>>((float*)p, n, (float)value);
+    AMX_HIP(hipGetLastError());
+}
+
+void launch_coef(const double* diag, const uint32_t* cdofs, uint32_t nc, double penalty, double* coef64, hipStream_t s)
+{
+    if (nc == 0) return;
+    coef_kernel<<<blocks_for(nc, 256), 256, 0, s>>>(diag, cdofs, nc, penalty, coef64);
+    AMX_HIP(hipGetLastError());
+}
+
+void launch_set_entries(double* u, const uint32_t* ids, uint32_t n, double value, hipStream_t s)
+{
+    if (n == 0) return;
+    set_entries_kernel<<<blocks_for(n, 256), 256, 0, s>>>(u, ids, n, value);
+    AMX_HIP(hipGetLastError());
+}
+
+void launch_halo_accumulate(double* v, const double* recv, const uint32_t* dst, const uint32_t* off,
+                            const uint32_t* src, uint32_t n_unique, hipStream_t s)
+{
+    if (n_unique == 0) return;
+    halo_accumulate_kernel<<<blocks_for(n_unique, 256), 256, 0, s>>>(v, recv, dst, off, src, n_unique);
+    AMX_HIP(hipGetLastError());
+}
+
+void launch_d2f(const double* a, float* b, uint64_t n, hipStream_t s)
+{
+    if (n == 0) return;
+    d2f_kernel<<<blocks_for(n, 256), 256, 0, s>>>(a, b, n);
+    AMX_HIP(hipGetLastError());
+}
+
+void launch_f2d(const float* a, double* b, uint64_t n, hipStream_t s)
+{
+    if (n == 0) return;
+    f2d_kernel<<<blocks_for(n, 256), 256, 0, s>>>(a, b, n);
+    AMX_HIP(hipGetLastError());
+}
+
+}  // namespace amx

@@ -1,0 +1,72 @@
+// amx_kernels.hpp -- host-side launchers of the gfx950 kernels (amx_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace amx {
+
+// Element-matrix storage types on the device.
+enum class KType : int { F64 = 0, F32 = 1, BF16 = 2 };
+
+// Device layout of one element group's matrices ("column-packed tiles"):
+// a wave processes a tile of G elements with n DoFs each (lane = g*n + r);
+// pack j of the tile holds, for every lane, the P consecutive columns
+// j*P .. j*P+P-1 of row r, so one pack is one contiguous G*n*P-value load of
+// the whole wave.  Offset of (slot, r, c), slot = t*G + g:
+//   ((t*(n/P) + c/P)*(G*n) + g*n + r)*P + c%P.
+struct Layout {
+    uint32_t n = 0, G = 1, P = 1;
+    bool specialized = false;  // a template kernel exists for (n, G, P, type)
+};
+
+Layout choose_layout(uint32_t n, KType kt);
+size_t ktype_size(KType kt);
+
+// ---- set-up kernels -----------------------------------------------------
+// row-major FP64 element matrices (group element order) -> tiled layout of kt.
+// slot_elem[slot] = index into src (UINT32_MAX for padding slots -> zeros).
+void launch_relayout_from_rowmajor(const double* src, const uint32_t* slot_elem, uint32_t n_slots,
+                                   const Layout& L, KType kt, void* dst, hipStream_t s);
+// K_e = scale[elem] * k_ref (row-major n x n) -> tiled layout of kt.
+void launch_relayout_scaled(const double* k_ref, const double* scale, const uint32_t* slot_elem,
+                            uint32_t n_slots, const Layout& L, KType kt, void* dst, hipStream_t s);
+// diagonal entries K_e(r,r) of every slot row from the tiled FP64 storage into ve.
+void launch_extract_diag(const double* K, const Layout& L, uint32_t n_slots, double* ve, hipStream_t s);
+
+// ---- matvec kernels ----------------------------------------------------
+struct EbeArgs {
+    const void* K;          // tiled storage
+    const uint32_t* map;    // [slot][r] local DoF ids, bit 31 = zero on gather (identity BC)
+    const void* u;
+    void* out;              // ve (deterministic) or v (atomic)
+    uint64_t ve_base;       // first ve row of this group
+    uint32_t tile_begin, n_tiles;
+    const uint32_t* skip = nullptr;  // device flag: nonzero -> kernel exits (converged solver)
+};
+// vt_double: vectors/accumulation in FP64 (only with KType::F64), else FP32.
+void launch_ebe(const Layout& L, KType kt, bool vt_double, bool masked, bool atomic, const EbeArgs& a,
+                hipStream_t s);
+
+// v[d] = sum_{q in [off[d], off[d+1])} ve[pos[q]] (fixed order), then the BC
+// form: bc 0 none; 1 identity (flag[d] -> v[d] = u[d]); 2 penalty
+// (flag[d] -> v[d] += coef[d] * u[d]).
+void launch_gather(bool vt_double, int bc, const uint32_t* off, const uint32_t* pos, const void* ve,
+                   const void* u, const uint8_t* flag, const double* coef, void* v, uint32_t n, hipStream_t s,
+                   const uint32_t* skip = nullptr);
+
+// constrained rows after an atomic scatter or a halo accumulation
+void launch_bc_fixup(bool vt_double, int bc, const uint32_t* cdofs, uint32_t nc, const double* coef,
+                     const void* u, void* v, hipStream_t s);
+void launch_fill(void* p, uint64_t n, bool vt_double, double value, hipStream_t s);
+void launch_coef(const double* diag, const uint32_t* cdofs, uint32_t nc, double penalty, double* coef64,
+                 hipStream_t s);
+void launch_set_entries(double* u, const uint32_t* ids, uint32_t n, double value, hipStream_t s);
+// halo accumulate: v[dst[i]] += sum_{q in [off[i], off[i+1])} recv[src[q]], i < n_unique
+void launch_halo_accumulate(double* v, const double* recv, const uint32_t* dst, const uint32_t* off,
+                            const uint32_t* src, uint32_t n_unique, hipStream_t s);
+void launch_d2f(const double* a, float* b, uint64_t n, hipStream_t s);
+void launch_f2d(const float* a, double* b, uint64_t n, hipStream_t s);
+
+}  // namespace amx

@@ -1,0 +1,127 @@
+// amx_op.hpp -- the operator handle (internal).
+#pragma once
+
+#include <memory>
+
+#include "amx_internal.hpp"
+#include "amx_kernels.hpp"
+#include "amx_solver.hpp"
+
+namespace amx {
+
+constexpr uint32_t kPadSlot = 0xffffffffu;
+constexpr double kPenaltyFactor = 100.0;  // PENALTY_FACTOR, include/aMat.hpp:68
+
+// Elements sharing a DoF count n.  Slots are the positions of the group's
+// device arrays: independent elements first (ascending id), padded to a whole
+// tile, then the dependent ones, padded again.
+struct Group {
+    uint32_t n = 0;
+    std::vector<uint32_t> elems;      // element ids, ascending (staging order)
+    std::vector<double> stage;        // row-major n x n per element (staging order)
+    Layout L[3];                      // per KType
+    uint32_t G = 1;
+    std::vector<uint32_t> slot_elem;  // slot -> index into elems, kPadSlot for padding
+    std::vector<uint32_t> elem_slot;  // index into elems -> slot
+    uint32_t n_slots = 0, n_tiles = 0, tiles_phase0 = 0;
+    uint64_t ve_base = 0;             // first ve row
+    DevBuf K[3];                      // tiled storage per KType (F64 always)
+    DevBuf map;                       // [slot][r] local ids | identity-BC flag
+};
+
+struct Op {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int matfree_type = AMX_MATFREE_GPU_PURE;
+    uint32_t n_streams = 1;
+    int bc_meth = AMX_BC_IMATRIX;
+    int bc_operator = AMX_BCOP_IDENTITY;
+    int scatter = AMX_SCATTER_DETERMINISTIC;
+    int prec_mask = AMX_PREC_FP64;
+    bool penalty_set = false;
+    double penalty_value = 0.0;
+    double trace = 0.0;
+    bool finalized = false;
+
+    // copied from the maps
+    uint32_t rank = 0, size = 1;
+    uint32_t n_elem = 0, n_total = 0, n_owned = 0, n_pre = 0, n_post = 0;
+    uint64_t dof_lo = 0;
+    std::vector<uint32_t> dpe;
+    std::vector<uint64_t> eoff;
+    std::vector<uint32_t> local_map;
+    std::vector<uint8_t> dependent;      // per element
+    std::vector<uint32_t> indep_list, dep_list;
+    bool has_bdr = false;
+    std::vector<uint32_t> bdr_map;       // per element entry
+    std::vector<double> presval_map;
+    std::vector<uint32_t> owned_c_local; // local ids of Maps::ownedConstrainedDofs (input order)
+    std::vector<double> owned_c_vals;
+    std::vector<uint32_t> owned_free_local;
+    std::vector<uint32_t> send_ids;      // ghost_send_end targets (concatenated over ranks)
+
+    // element matrices
+    std::vector<int32_t> elem_group, elem_index;
+    std::vector<uint32_t> elem_bdim;     // blocks_dim last used (0 = never set)
+    bool scaled = false;
+    std::vector<double> kref;            // row-major, scaled mode
+    std::vector<double> scale;           // per element, scaled mode
+    std::vector<Group> groups;
+
+    // device tables
+    uint64_t ve_rows = 0;
+    DevBuf ve, off, pos, flag, coef64, cdofs, diag, diag_bc_ids, ghost_c_ids;
+    uint32_t n_cdofs = 0, n_diag_bc = 0, n_ghost_c = 0;
+    DevBuf halo_dst, halo_off, halo_src;
+    uint32_t n_halo_unique = 0;
+    DevBuf scratch_u, scratch_v;         // host-array API staging
+    // solver workspace (allocated on first solve)
+    DevBuf ws_r, ws_z, ws_p, ws_q, ws_dinv, ws_partials, ws_sc, ws_x32, ws_b32, ws_r32, ws_z32, ws_p32, ws_q32;
+    CgScalars* host_sc = nullptr;        // pinned copy of the device scalars
+    uint64_t device_bytes = 0;
+
+    Op() = default;
+    Op(const Op&) = delete;
+    Op& operator=(const Op&) = delete;
+    ~Op()
+    {
+        if (host_sc) (void)hipHostFree(host_sc);
+    }
+    double penalty() const { return penalty_set ? penalty_value : kPenaltyFactor * trace; }
+    hipStream_t pick(void* s) const { return s ? static_cast<hipStream_t>(s) : stream; }
+    bool has(int prec_bit) const { return (prec_mask & prec_bit) != 0; }
+};
+
+// RAII: make the handle's device current, restore the previous one on exit.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) AMX_HIP(hipSetDevice(dev));
+    }
+    ~DeviceGuard()
+    {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+// launch pieces shared with the solvers
+// bc: 0 none, 1 identity, 2 penalty.  phase: 0 independent tiles, 1 dependent
+// tiles, 2 all tiles.  vt_double: FP64 vectors and storage, else FP32 vectors
+// with storage kt (F32 or BF16).
+// skip: optional device flag; a nonzero value turns the launches into no-ops.
+void ebe_launch(Op& op, KType kt, bool vt_double, bool masked, int phase, const void* u, void* v, hipStream_t s,
+                const uint32_t* skip = nullptr);
+void gather_launch(Op& op, bool vt_double, int bc, const void* u, void* v, hipStream_t s,
+                   const uint32_t* skip = nullptr);
+// complete one-rank operator application: v = A u with BC form bc
+void apply(Op& op, KType kt, bool vt_double, int bc, const void* u, void* v, hipStream_t s,
+           const uint32_t* skip = nullptr);
+
+}  // namespace amx
+
+struct amx_op {
+    amx::Op op;
+};

@@ -1,0 +1,194 @@
+// amx_solver.cpp -- device Jacobi-PCG and FP64 iterative refinement around
+// the EbE operator (the KSPCG + PCJACOBI solve of examples/include/solve.hpp:45-65,
+// run entirely on the GPU).
+//
+// The iteration runs without host round trips: scalars stay on the device and
+// the host only polls the converged flag every few iterations (launches after
+// convergence are no-ops, the matvec kernels included).
+#include <chrono>
+#include <cmath>
+
+#include "amx_op.hpp"
+
+using amx::DevBuf;
+using amx::fail;
+using amx::guarded;
+using amx::KType;
+using amx::Op;
+
+namespace amx {
+
+namespace {
+
+void ensure(DevBuf& b, size_t bytes)
+{
+    if (b.bytes < bytes) b.alloc(bytes);
+}
+
+void ensure_ws(Op& op)
+{
+    const size_t n = std::max<uint32_t>(op.n_total, 1);
+    for (DevBuf* b : {&op.ws_r, &op.ws_z, &op.ws_p, &op.ws_q, &op.ws_dinv}) ensure(*b, n * sizeof(double));
+    for (DevBuf* b : {&op.ws_x32, &op.ws_b32, &op.ws_r32, &op.ws_z32, &op.ws_p32, &op.ws_q32}) ensure(*b, n * sizeof(float));
+    ensure(op.ws_partials, 3 * 1024 * sizeof(double));
+    ensure(op.ws_sc, 2 * sizeof(CgScalars));
+    if (!op.host_sc) {
+        void* p = nullptr;
+        AMX_HIP(hipHostMalloc(&p, 2 * sizeof(CgScalars), hipHostMallocDefault));
+        op.host_sc = static_cast<CgScalars*>(p);
+    }
+}
+
+void set_scalars(Op& op, CgScalars* dsc, double rtol, int max_it, hipStream_t s)
+{
+    CgScalars h{};
+    h.tol2 = rtol * rtol;
+    h.max_it = (uint32_t)max_it;
+    h.counter = 0;
+    *op.host_sc = h;
+    AMX_HIP(hipMemcpyAsync(dsc, op.host_sc, sizeof(CgScalars), hipMemcpyHostToDevice, s));
+    AMX_HIP(hipStreamSynchronize(s));
+}
+
+CgScalars read_scalars(Op& op, const CgScalars* dsc, hipStream_t s)
+{
+    AMX_HIP(hipMemcpyAsync(op.host_sc, dsc, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
+    AMX_HIP(hipStreamSynchronize(s));
+    return *op.host_sc;
+}
+
+// Jacobi diagonal with the BC values (MatGetDiagonal_mf) -> 1/diag
+void build_dinv(Op& op, amx_op* h, hipStream_t s)
+{
+    const int rc = amx_op_get_diagonal_dev(h, op.ws_q.as<double>(), s);
+    if (rc != AMX_SUCCESS) fail(rc, "diagonal failed");
+    invert(op.ws_q.as<double>(), op.ws_dinv.as<double>(), op.n_total, s);
+}
+
+// PCG on A x = b (x starts at 0), vectors of type f64 ? double : float, the
+// operator with storage kt and the handle's BC form.  Returns iterations.
+CgScalars pcg(Op& op, KType kt, bool f64, const void* b, void* x, void* r, void* z, void* p, void* q, double rtol,
+              int max_it, CgScalars* dsc, hipStream_t s)
+{
+    const uint32_t n = op.n_total;
+    double* partials = op.ws_partials.as<double>();
+    const double* dinv = op.ws_dinv.as<double>();
+    set_scalars(op, dsc, rtol, max_it, s);
+    cg_init(f64, b, dinv, x, r, z, p, n, partials, dsc, s);
+    // poll every `chunk` iterations: about 1/8 of the iterations of a small
+    // solve, every other one of a large solve
+    const uint32_t chunk = n > (1u << 21) ? 2 : (n > (1u << 17) ? 4 : 8);
+    const uint32_t* skip = &dsc->done;
+    for (int launched = 0;;) {
+        for (uint32_t k = 0; k < chunk; k++) {
+            apply(op, kt, f64, op.bc_operator, p, q, s, skip);
+            cg_pq(f64, p, q, n, partials, dsc, s);
+            cg_update(f64, x, r, z, p, q, dinv, n, partials, dsc, s);
+            cg_p(f64, p, z, n, dsc, s);
+        }
+        launched += (int)chunk;
+        const CgScalars sc = read_scalars(op, dsc, s);
+        if (sc.done || launched >= max_it) return sc;
+    }
+}
+
+}  // namespace
+
+}  // namespace amx
+
+extern "C" {
+
+int amx_cg_solve_dev(amx_op* h, const double* b, double* x, double rtol, int max_it, amx_solve_stats* stats,
+                     void* stream)
+{
+    return guarded(__func__, [&] {
+        AMX_REQUIRE(h && b && x, AMX_INVALID_ARGUMENT, "NULL argument");
+        AMX_REQUIRE(rtol > 0.0 && max_it > 0, AMX_INVALID_ARGUMENT, "rtol and max_it must be positive");
+        Op& op = h->op;
+        AMX_REQUIRE(op.finalized, AMX_INVALID_STATE, "finalize() has not been called");
+        AMX_REQUIRE(op.size == 1, AMX_NOT_IMPLEMENTED, "device solver is one-rank");
+        amx::DeviceGuard dg(op.device);
+        hipStream_t s = op.pick(stream);
+        const auto t0 = std::chrono::steady_clock::now();
+        amx::ensure_ws(op);
+        amx::build_dinv(op, h, s);
+        amx::CgScalars* dsc = op.ws_sc.as<amx::CgScalars>();
+        const amx::CgScalars sc = amx::pcg(op, KType::F64, true, b, x, op.ws_r.p, op.ws_z.p, op.ws_p.p, op.ws_q.p,
+                                           rtol, max_it, dsc, s);
+        // true residual ||b - A x|| / ||b|| in FP64
+        amx::apply(op, KType::F64, true, op.bc_operator, x, op.ws_q.p, s);
+        amx::residual(b, op.ws_q.as<double>(), op.ws_r.as<double>(), op.n_total, op.ws_partials.as<double>(), dsc, s);
+        const amx::CgScalars fin = amx::read_scalars(op, dsc, s);
+        const double rel = fin.bb > 0 ? std::sqrt(fin.rr / fin.bb) : 0.0;
+        if (stats) {
+            stats->iterations = (int32_t)sc.iter;
+            stats->outer_iterations = 0;
+            stats->converged = rel <= rtol * 1.0000001 || fin.bb == 0.0;
+            stats->precision = AMX_PREC_FP64;
+            stats->rel_residual = rel;
+            stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        AMX_REQUIRE(!sc.breakdown, AMX_NOT_CONVERGED, "CG breakdown: p^T A p <= 0 (operator not SPD)");
+    });
+}
+
+int amx_ir_cg_solve_dev(amx_op* h, int precision, const double* b, double* x, double rtol, double inner_rtol,
+                        int max_outer, int max_inner, amx_solve_stats* stats, void* stream)
+{
+    return guarded(__func__, [&] {
+        AMX_REQUIRE(h && b && x, AMX_INVALID_ARGUMENT, "NULL argument");
+        AMX_REQUIRE(precision == AMX_PREC_FP32 || precision == AMX_PREC_BF16, AMX_INVALID_ARGUMENT,
+                    "inner precision must be FP32 or BF16");
+        AMX_REQUIRE(rtol > 0.0 && inner_rtol > 0.0 && inner_rtol < 1.0 && max_outer > 0 && max_inner > 0,
+                    AMX_INVALID_ARGUMENT, "bad tolerances or iteration limits");
+        Op& op = h->op;
+        AMX_REQUIRE(op.finalized, AMX_INVALID_STATE, "finalize() has not been called");
+        AMX_REQUIRE(op.has(precision), AMX_INVALID_STATE, "inner precision was not built at finalize");
+        AMX_REQUIRE(op.size == 1, AMX_NOT_IMPLEMENTED, "device solver is one-rank");
+        amx::DeviceGuard dg(op.device);
+        hipStream_t s = op.pick(stream);
+        const auto t0 = std::chrono::steady_clock::now();
+        amx::ensure_ws(op);
+        amx::build_dinv(op, h, s);
+        const KType kt = precision == AMX_PREC_FP32 ? KType::F32 : KType::BF16;
+        amx::CgScalars* outer = op.ws_sc.as<amx::CgScalars>();
+        amx::CgScalars* inner = outer + 1;
+        const uint32_t n = op.n_total;
+        double* partials = op.ws_partials.as<double>();
+        // x = 0, r = b
+        amx::launch_fill(x, n, true, 0.0, s);
+        amx::launch_fill(op.ws_q.p, n, true, 0.0, s);
+        amx::residual(b, op.ws_q.as<double>(), op.ws_r.as<double>(), n, partials, outer, s);
+        int total_inner = 0, outer_it = 0;
+        double rel = 1.0;
+        bool breakdown = false;
+        for (;;) {
+            const amx::CgScalars o = amx::read_scalars(op, outer, s);
+            rel = o.bb > 0 ? std::sqrt(o.rr / o.bb) : 0.0;
+            if (rel <= rtol || outer_it >= max_outer) break;
+            // inner solve A_low d = r / ||r|| (FP32 vectors, FP64 scalars)
+            amx::scale_to_f32(op.ws_r.as<double>(), op.ws_b32.as<float>(), n, outer, s);
+            const amx::CgScalars in = amx::pcg(op, kt, false, op.ws_b32.p, op.ws_x32.p, op.ws_r32.p, op.ws_z32.p,
+                                               op.ws_p32.p, op.ws_q32.p, inner_rtol, max_inner, inner, s);
+            total_inner += (int)in.iter;
+            breakdown = breakdown || in.breakdown;
+            // x += ||r|| d ; r = b - A x with the FP64 operator
+            amx::correct(x, op.ws_x32.as<float>(), n, outer, s);
+            amx::apply(op, KType::F64, true, op.bc_operator, x, op.ws_q.p, s);
+            amx::residual(b, op.ws_q.as<double>(), op.ws_r.as<double>(), n, partials, outer, s);
+            outer_it++;
+            if (in.iter == 0) break;  // no progress possible
+        }
+        if (stats) {
+            stats->iterations = total_inner;
+            stats->outer_iterations = outer_it;
+            stats->converged = rel <= rtol;
+            stats->precision = precision;
+            stats->rel_residual = rel;
+            stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        AMX_REQUIRE(!breakdown, AMX_NOT_CONVERGED, "inner CG breakdown (low-precision operator not SPD)");
+    });
+}
+
+}  // extern "C"

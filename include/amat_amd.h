@@ -1,0 +1,337 @@
+/*
+ * amat_amd.h -- C ABI of the MI355X-native assembly-free element-by-element
+ * (EbE) operator  v = sum_e P_e^T K_e P_e u.
+ *
+ * This is the drop-in boundary for aMat's GPU methods (MATFREE_TYPE 3/4/5,
+ * include/enums.hpp:56 of paralab/aMat).  Every entry point names the
+ * reference interface it replaces (file:line in the paralab/aMat tree).  Plain
+ * pointers and sizes only; no C++ or torch types cross this boundary.
+ *
+ * Types follow the reference instantiation <DT,GI,LI> =
+ * <double, unsigned long, unsigned int> (examples/src/example03a.cpp:423-426):
+ * global ids are uint64_t, local ids uint32_t, values double.
+ *
+ * Ownership: every set_* call copies its inputs; no pointer passed in is kept
+ * after the call returns.  Device buffers belong to the handle.
+ * Threading: a handle is used by one host thread at a time; distinct handles
+ * are independent.  Device entry points (*_dev) are asynchronous on the given
+ * HIP stream (NULL = the handle's own stream).
+ * Errors: every call returns an int status (AMX_SUCCESS = 0); the values below
+ * AMX_INVALID_ARGUMENT mirror par::Error (include/enums.hpp:24-37).  The
+ * message of the last failure on the calling thread is in amx_last_error().
+ * Nothing here calls exit().
+ */
+#ifndef AMAT_AMD_H
+#define AMAT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+/* par::Error, include/enums.hpp:24-37 */
+enum amx_status {
+    AMX_SUCCESS = 0,
+    AMX_INDEX_OUT_OF_BOUNDS = 1,
+    AMX_UNKNOWN_ELEMENT_TYPE = 2,
+    AMX_UNKNOWN_ELEMENT_STATUS = 3,
+    AMX_NULL_L2G_MAP = 4,
+    AMX_GHOST_NODE_NOT_FOUND = 5,
+    AMX_UNKNOWN_MAT_TYPE = 6,
+    AMX_WRONG_COMMUNICATION = 7,
+    AMX_UNKNOWN_DOF_TYPE = 8,
+    AMX_UNKNOWN_CONSTRAINT = 9,
+    AMX_UNKNOWN_BC_METH = 10,
+    AMX_NOT_IMPLEMENTED = 11,
+    AMX_GLOBAL_DOF_ID_NOT_FOUND = 12,
+    /* additions of this library */
+    AMX_INVALID_ARGUMENT = 100,
+    AMX_DEVICE_ERROR = 101,
+    AMX_OUT_OF_MEMORY = 102,
+    AMX_INVALID_STATE = 103,
+    AMX_NOT_CONVERGED = 104,
+    AMX_REJECTED = 105
+};
+
+/* par::BC_METH, include/enums.hpp:20-22 */
+enum amx_bc_meth { AMX_BC_IMATRIX = 0, AMX_BC_PENALTY = 1 };
+
+/* par::MATFREE_TYPE, include/enums.hpp:56.  This library runs 3/4/5 on the
+ * GPU; 1 and 2 are the reference's CPU loops and return AMX_NOT_IMPLEMENTED. */
+enum amx_matfree_type {
+    AMX_MATFREE_UNDEFINED = 0,
+    AMX_MATFREE_QUASI_FREE = 1,
+    AMX_MATFREE_FREE = 2,
+    AMX_MATFREE_GPU_PURE = 3,
+    AMX_MATFREE_GPU_OVER_CPU = 4,
+    AMX_MATFREE_GPU_OVER_GPU = 5
+};
+
+/* storage precision of the element matrices (bit mask, amx_op_set_options) */
+enum amx_precision { AMX_PREC_FP64 = 1, AMX_PREC_FP32 = 2, AMX_PREC_BF16 = 4 };
+
+/* how element vectors are summed into v */
+enum amx_scatter {
+    AMX_SCATTER_DETERMINISTIC = 0, /* node -> (element, local id) transposed gather, fixed order */
+    AMX_SCATTER_ATOMIC = 1         /* FP64/FP32 hardware atomics, order not fixed */
+};
+
+/* boundary-condition form applied by amx_op_matmult_* */
+enum amx_bc_operator {
+    AMX_BCOP_NONE = 0,        /* plain K u */
+    AMX_BCOP_IDENTITY = 1,    /* aMatFree::MatMult_mf identity rows, include/aMatFree.hpp:3893-3941 */
+    AMX_BCOP_PENALTY_DIAG = 2 /* aMatBased::apply_bc_mat penalty: K_cc := p, include/aMatBased.hpp:332-347 */
+};
+
+/* element-matrix storage order of the caller's arrays */
+enum amx_layout { AMX_ROW_MAJOR = 0, AMX_COL_MAJOR = 1 };
+
+/* Message of the last failed call on this thread ("" if none).  Copies at most
+ * len-1 bytes plus a terminating NUL; returns the full message length. */
+size_t amx_last_error(char* buf, size_t len);
+
+/* Library version string, e.g. "amat_amd 0.1.0 gfx950". */
+const char* amx_version(void);
+
+/* Number of visible HIP devices (0 when none); never fails. */
+int amx_device_count(void);
+
+/* ------------------------------------------------------------------ maps
+ * par::Maps<DT,GI,LI> (include/maps.hpp).  One handle per rank.  In the
+ * reference the counts of all ranks come from MPI_Allgather (maps.hpp:547) and
+ * the ghost-id exchange is an MPI_Alltoallv (maps.hpp:738); here the caller
+ * brings the counts (rank_dof_counts) and moves the request lists with its own
+ * transport (amx_maps_get_ghost_requests / amx_maps_set_peer_requests).  With
+ * one rank no exchange is needed. */
+typedef struct amx_maps amx_maps;
+
+typedef struct amx_maps_info {
+    uint32_t rank, size;
+    uint32_t n_elems;         /* Maps::get_NumElems */
+    uint32_t n_owned;         /* Maps::get_NumDofs */
+    uint32_t n_pre_ghost;     /* Maps::get_NumPreGhostDofs */
+    uint32_t n_post_ghost;    /* Maps::get_NumPostGhostDofs */
+    uint32_t n_total;         /* Maps::get_NumDofsTotal */
+    uint32_t n_entries;       /* sum of dofs_per_element */
+    uint32_t max_dofs_per_elem;
+    uint32_t n_independent, n_dependent;
+    uint32_t n_owned_constraints, n_owned_free;
+    uint32_t n_send, n_recv;  /* lengths of the send/recv id lists */
+    uint32_t has_bdr_map, has_peer_lists;
+    uint64_t owned_begin, owned_end, n_global;
+} amx_maps_info;
+
+/* Maps::Maps(MPI_Comm) (include/maps.hpp:80) */
+int amx_maps_create(uint32_t rank, uint32_t size, amx_maps** out);
+int amx_maps_destroy(amx_maps* maps);
+
+/* Maps::set_map (include/maps.hpp:398-451) + buildScatterMap (:519-826).
+ * element_to_rank_map: the LI** map flattened element after element
+ * (element e holds dofs_per_element[e] entries).  owned range inclusive.
+ * rank_dof_counts[size]: owned DoF count of every rank (NULL when size == 1). */
+int amx_maps_set_map(amx_maps* maps, uint32_t n_elements_on_rank,
+                     const uint32_t* element_to_rank_map, const uint32_t* dofs_per_element,
+                     uint32_t n_all_dofs_on_rank, const uint64_t* rank_to_global_map,
+                     uint64_t owned_global_dof_range_begin, uint64_t owned_global_dof_range_end,
+                     uint64_t n_global_dofs, const uint32_t* rank_dof_counts);
+
+/* sendBuf of buildScatterMap's Alltoallv (include/maps.hpp:716-739): ghost
+ * global ids grouped by owner rank.  counts[size]; gids[n_pre+n_post]. */
+int amx_maps_get_ghost_requests(const amx_maps* maps, uint32_t* counts, uint64_t* gids);
+
+/* recvBuf of that Alltoallv: the ids peers requested from this rank, grouped
+ * by requesting rank (counts[size]).  Builds m_uivSendDofIds and the
+ * send/recv counts and offsets (include/maps.hpp:741-785). */
+int amx_maps_set_peer_requests(amx_maps* maps, const uint32_t* counts, const uint64_t* gids);
+
+/* Maps::set_bdr_map (include/maps.hpp:828-880).  Same outputs as the
+ * reference's linear scan, found through a hash of the constraint list (first
+ * occurrence wins, as the reference's `break`). */
+int amx_maps_set_bdr_map(amx_maps* maps, const uint64_t* constrained_dofs,
+                         const double* prescribed_values, uint32_t n_constraints);
+
+int amx_maps_get_info(const amx_maps* maps, amx_maps_info* info);
+/* getters (include/maps.hpp:85-276); arrays are sized from amx_maps_info */
+int amx_maps_get_local_map(const amx_maps* maps, uint32_t* out);      /* n_entries, get_LocalMap */
+int amx_maps_get_global_map(const amx_maps* maps, uint64_t* out);     /* n_entries, get_Map */
+int amx_maps_get_local2global(const amx_maps* maps, uint64_t* out);   /* n_total, get_Local2Global */
+int amx_maps_get_localid2rank(const amx_maps* maps, uint32_t* out);   /* n_total, get_localID2Rank */
+int amx_maps_get_bdr_map(const amx_maps* maps, uint32_t* mask, double* presval); /* n_entries each */
+int amx_maps_get_owned_constraints(const amx_maps* maps, uint64_t* dofs, double* values);
+int amx_maps_get_owned_free(const amx_maps* maps, uint64_t* dofs);
+int amx_maps_get_element_lists(const amx_maps* maps, uint32_t* independent, uint32_t* dependent);
+/* per-rank counts/offsets (size entries each) and id lists (n_send / n_recv) */
+int amx_maps_get_send_lists(const amx_maps* maps, uint32_t* counts, uint32_t* offsets, uint32_t* ids);
+int amx_maps_get_recv_lists(const amx_maps* maps, uint32_t* counts, uint32_t* offsets);
+int amx_maps_get_dof_scan(const amx_maps* maps, uint64_t* scan);     /* size, get_LocalDofScan */
+
+/* ---------------------------------------------------------------- operator
+ * par::aMatFree<DT,GI,LI> (include/aMatFree.hpp:25-353) for methods 3/4/5,
+ * replacing aMatGpu (include/aMatGpu.hpp:21-591). */
+typedef struct amx_op amx_op;
+
+typedef struct amx_op_info {
+    uint32_t n_elems, n_total, n_owned, n_pre_ghost;
+    uint32_t n_groups;          /* element groups by DoFs-per-element */
+    uint32_t finalized;
+    int32_t matfree_type, bc_meth, bc_operator, scatter_mode, precision_mask;
+    double trace;               /* m_dtTraceK, include/aMatFree.hpp:748-785 */
+    double penalty;             /* value placed on constrained diagonals */
+    uint64_t device_bytes;      /* device memory held by the handle */
+    uint64_t k_bytes_fp64, k_bytes_fp32, k_bytes_bf16; /* element-matrix storage */
+    uint64_t alg_bytes_fp64;    /* E(n^2 8 + 4n) + 16 N: one FP64 matvec's algorithmic HBM bytes */
+    uint64_t flops;             /* 2 sum_e n_e^2 per matvec */
+} amx_op_info;
+
+/* aMatFree(Maps&, BC_METH) (include/aMatFree.hpp:405-439).  Copies the maps it
+ * needs; `maps` may be destroyed afterwards.  device = HIP ordinal. */
+int amx_op_create(const amx_maps* maps, int bc_meth, int device, amx_op** out);
+int amx_op_destroy(amx_op* op);
+
+/* aMat::set_matfree_type (include/aMat.hpp:184) */
+int amx_op_set_matfree_type(amx_op* op, int matfree_type);
+/* aMatFree::set_num_streams (include/aMatFree.hpp:230-237): number of HIP
+ * streams used by the two-phase (independent/dependent) launch; >= 1 */
+int amx_op_set_num_streams(amx_op* op, uint32_t n_streams);
+
+/* aMatFree::set_element_matrix -> copy_element_matrix
+ * (include/aMatFree.hpp:854-953): block (block_i, block_j) of element eid,
+ * n_b x n_b with n_b = dofs_per_element[eid] / blocks_dim, in `layout` order. */
+int amx_op_set_element_matrix(amx_op* op, uint32_t eid, const double* e_mat, int layout,
+                              uint32_t block_i, uint32_t block_j, uint32_t blocks_dim);
+/* Bulk form of the above for count consecutive elements with equal DoF
+ * counts (one n x n matrix each, back to back). */
+int amx_op_set_element_matrices(amx_op* op, uint32_t first_eid, uint32_t count,
+                                const double* e_mats, int layout);
+/* K_e = scale[e] * k_ref for every element (all elements share n): the
+ * congruent structured meshes of the examples with per-element stiffness. */
+int amx_op_set_element_matrices_scaled(amx_op* op, const double* k_ref, int layout,
+                                       const double* scale);
+
+/* precision_mask: AMX_PREC_* storages to build at finalize (FP64 always kept);
+ * scatter_mode: AMX_SCATTER_*. */
+int amx_op_set_options(amx_op* op, int precision_mask, int scatter_mode);
+/* value placed on constrained diagonals by AMX_BCOP_PENALTY_DIAG; by default
+ * PENALTY_FACTOR (100, include/aMat.hpp:68) times the trace. */
+int amx_op_set_penalty(amx_op* op, double penalty);
+/* which BC form amx_op_matmult_* applies (default AMX_BCOP_IDENTITY, what
+ * aMatFree::MatMult_mf does whatever the BC method). */
+int amx_op_set_bc_operator(amx_op* op, int bc_operator);
+
+/* aMatFree::finalize (include/aMatFree.hpp:151-155, 656-785): uploads K in
+ * the device layout, builds the transposed gather lists, BC tables, trace
+ * (element order) and the assembled diagonal. */
+int amx_op_finalize(amx_op* op);
+int amx_op_get_info(const amx_op* op, amx_op_info* info);
+
+/* aMatFree::matvec (include/aMatFree.hpp:1681-1775): v = K u, no BCs, host
+ * arrays.  is_ghosted != 0: length n_total; else n_owned (ghost slots 0). */
+int amx_op_matvec(amx_op* op, double* v, const double* u, int is_ghosted);
+
+/* aMatFree::MatMult_mf array semantics (include/aMatFree.hpp:3893-3941) with
+ * the configured BC form, owned-length host arrays (one rank). */
+int amx_op_matmult_mf(amx_op* op, double* vv, const double* uu);
+
+/* Device forms on ghosted-length device arrays, asynchronous on `stream`.
+ * matvec_dev: v = K u.  matmult_dev: the configured BC form.  One rank:
+ * ghost slots are empty.  Several ranks: see amx_op_matvec_begin_dev. */
+int amx_op_matvec_dev(amx_op* op, double* v, const double* u, void* stream);
+int amx_op_matmult_dev(amx_op* op, double* v, const double* u, void* stream);
+
+/* Two-phase form of matvec_ghosted_gpuOverGpu (include/aMatFree.hpp:3858-3889):
+ * begin launches the independent elements (they read owned u only); the caller
+ * then fills u's ghost slots (amx_op_halo_import_dev); end launches the
+ * dependent elements and the gather.  bc_operator as amx_op_matmult_dev when
+ * apply_bc != 0; ghost slots of v then hold contributions for their owners
+ * (amx_op_halo_export_dev / amx_op_halo_accumulate_dev), after which
+ * amx_op_bc_fixup_dev applies the constrained rows. */
+int amx_op_matvec_begin_dev(amx_op* op, double* v, const double* u, int apply_bc, void* stream);
+int amx_op_matvec_end_dev(amx_op* op, double* v, const double* u, int apply_bc, void* stream);
+/* ghost_receive_end memcpy (include/aMatFree.hpp:1536-1539): halo[n_pre+n_post]
+ * (pre then post) -> ghost slots of u.  Zeroes constrained ghosts when the
+ * identity BC form is active, as the owner's MatMult_mf does. */
+int amx_op_halo_import_dev(amx_op* op, double* u, const double* halo, void* stream);
+/* ghost slots of v -> halo[n_pre+n_post] (what ghost_send_begin sends) */
+int amx_op_halo_export_dev(amx_op* op, const double* v, double* halo, void* stream);
+/* ghost_send_end accumulate (include/aMatFree.hpp:1662-1669):
+ * v[send_ids[i]] += recv[i] in the reference's loop order (deterministic). */
+int amx_op_halo_accumulate_dev(amx_op* op, double* v, const double* recv, void* stream);
+/* constrained owned rows after accumulation (MatMult_mf :3929-3933 or penalty) */
+int amx_op_bc_fixup_dev(amx_op* op, double* v, const double* u, void* stream);
+
+/* Mixed-precision forms: K in FP32 or BF16 (must have been built at
+ * finalize), FP32 vectors, FP32 accumulation, configured BC form. */
+int amx_op_matmult_f32_dev(amx_op* op, int precision, float* v, const float* u, void* stream);
+
+/* aMatFree::MatGetDiagonal_mf (include/aMatFree.hpp:3944-3974):
+ * mat_get_diagonal_ghosted (:997-1056) + apply_bc_diagonal (:4309-4337).
+ * Host array, owned length.  _dev: device array, ghosted length. */
+int amx_op_get_diagonal(amx_op* op, double* diag);
+int amx_op_get_diagonal_dev(amx_op* op, double* diag, void* stream);
+
+/* aMatFree::apply_bc_rhs (include/aMatFree.hpp:4536-4700): rhs (owned length,
+ * host) -> rhs with constrained rows set (prescribed value, or penalty *
+ * prescribed value) and, for BC_IMATRIX, free rows reduced by K_fc U_c. */
+int amx_op_apply_bc_rhs(amx_op* op, double* rhs);
+
+/* Kernel timers, the GPU counterpart of aMatFree::get_timer
+ * (include/aMatFree.hpp:239-245, mv_time / gather_time): runs n_iter
+ * applications of the configured BC form on `stream` with hipEvents around
+ * the element-kernel launches and around the gather launch; returns the mean
+ * milliseconds of each and of the whole application.  Synchronizes. */
+int amx_op_time_kernels_dev(amx_op* op, int n_iter, double* v, const double* u, double* ms_element,
+                            double* ms_gather, double* ms_total, void* stream);
+
+/* Stream the handle launches on (hipStream_t as void*). */
+void* amx_op_stream(amx_op* op);
+int amx_op_synchronize(amx_op* op);
+
+/* --------------------------------------------------------------- solvers
+ * Device Krylov solvers around the operator (the KSPCG + PCJACOBI of
+ * examples/include/solve.hpp:45-65, on the GPU).  b, x: device arrays of
+ * length n_total (one rank).  Stop: ||b - A x||_2 <= rtol ||b||_2. */
+typedef struct amx_solve_stats {
+    int32_t iterations;         /* CG iterations (inner ones for IR) */
+    int32_t outer_iterations;   /* refinement steps (IR only) */
+    int32_t converged;
+    int32_t precision;          /* AMX_PREC_* of the inner operator */
+    double rel_residual;        /* true ||b - A x|| / ||b|| in FP64 at exit */
+    double seconds;
+} amx_solve_stats;
+
+/* FP64 Jacobi-PCG. */
+int amx_cg_solve_dev(amx_op* op, const double* b, double* x, double rtol, int max_it,
+                     amx_solve_stats* stats, void* stream);
+/* FP64 iterative refinement: true residual, alpha/beta, inner products and the
+ * penalty diagonal in FP64; inner Jacobi-PCG on the FP32 or BF16 operator to
+ * inner_rtol.  precision = AMX_PREC_FP32 or AMX_PREC_BF16. */
+int amx_ir_cg_solve_dev(amx_op* op, int precision, const double* b, double* x, double rtol,
+                        double inner_rtol, int max_outer, int max_inner,
+                        amx_solve_stats* stats, void* stream);
+
+/* ------------------------------------------------------- element kernels
+ * Element matrices of the examples (examples/src/ke_matrix.cpp,
+ * examples/src/fe_vector.cpp), FP64 on the host.  Row-major output. */
+/* integration<double>::gauss (examples/include/integration.hpp:81-113) */
+int amx_fem_gauss(uint32_t n, double* pts_n_wts);
+/* ke_hex8 (examples/src/ke_matrix.cpp:436-526) */
+int amx_fem_ke_hex8(const double* xe, uint32_t ngt, double* ke);
+/* ke_hex20_iso (examples/src/ke_matrix.cpp:1573-1740) */
+int amx_fem_ke_hex20_iso(const double* xe, double E, double nu, uint32_t ngt, double* ke);
+/* ke_tet10 (examples/src/ke_matrix.cpp:329-433) */
+int amx_fem_ke_tet10(const double* xe, uint32_t ngt, double* ke);
+/* fe_tet10 (examples/src/fe_vector.cpp:86-165) */
+int amx_fem_fe_tet10(const double* xe, const double* be, uint32_t ngt, double* fe);
+/* Many tet10 element matrices at once (OpenMP over elements):
+ * coords[n_elem*30] -> ke[n_elem*100]. */
+int amx_fem_ke_tet10_batch(uint32_t n_elem, const double* coords, uint32_t ngt, double* ke);
+int amx_fem_fe_tet10_batch(uint32_t n_elem, const double* coords, const double* be,
+                           uint32_t ngt, double* fe);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AMAT_AMD_H */

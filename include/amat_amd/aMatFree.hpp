@@ -1,0 +1,300 @@
+/*
+ * amat_amd/aMatFree.hpp -- header-only C++ drop-in for aMat's par::Maps,
+ * par::aMat and par::aMatFree (paralab/aMat include/maps.hpp, aMat.hpp,
+ * aMatFree.hpp) over the C ABI of amat_amd.h.
+ *
+ * Code written against the reference compiles against this header with the
+ * same names and arguments for the matrix-free GPU methods 3/4/5:
+ *
+ *   par::Maps<double, unsigned long, unsigned int> maps(comm_rank, comm_size);
+ *   maps.set_map(nelem, localDofMap, ndofs_per_element, numLocalDofs,
+ *                local2GlobalDofMap, start_global_dof, end_global_dof, ndofs_total);
+ *   maps.set_bdr_map(constrainedDofs, prescribedValues, n_constraints);
+ *   par::aMatFree<double, unsigned long, unsigned int> A(maps, par::BC_METH::BC_IMATRIX);
+ *   A.set_matfree_type(par::MATFREE_TYPE::GPU_PURE);
+ *   A.set_element_matrix(eid, ke, 0, 0, 1);      // any type with rows()/cols()/operator()(r,c)
+ *   A.finalize();
+ *   A.matvec(v, u, true);
+ *
+ * Differences: Maps takes (rank, size) instead of an MPI communicator (the
+ * halo transport belongs to the caller, see amx_maps_get_ghost_requests);
+ * failures throw par::amx_error carrying the par::Error code instead of
+ * exiting; methods 1/2 report NOT_IMPLEMENTED.  The PETSc shell hook
+ * (get_matrix) is compiled only with BUILD_WITH_PETSC.
+ */
+#ifndef AMAT_AMD_AMATFREE_HPP
+#define AMAT_AMD_AMATFREE_HPP
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "amat_amd.h"
+
+#ifdef BUILD_WITH_PETSC
+#include <petsc.h>
+#endif
+
+namespace par {
+
+enum class BC_METH { BC_IMATRIX = AMX_BC_IMATRIX, BC_PENALTY = AMX_BC_PENALTY };
+
+enum class Error {
+    SUCCESS = 0, INDEX_OUT_OF_BOUNDS, UNKNOWN_ELEMENT_TYPE, UNKNOWN_ELEMENT_STATUS, NULL_L2G_MAP,
+    GHOST_NODE_NOT_FOUND, UNKNOWN_MAT_TYPE, WRONG_COMMUNICATION, UNKNOWN_DOF_TYPE, UNKNOWN_CONSTRAINT,
+    UNKNOWN_BC_METH, NOT_IMPLEMENTED, GLOBAL_DOF_ID_NOT_FOUND
+};
+
+enum class MATFREE_TYPE { UNDEFINED = 0, QUASI_FREE = 1, FREE = 2, GPU_PURE = 3, GPU_OVER_CPU = 4, GPU_OVER_GPU = 5 };
+
+struct amx_error : std::runtime_error {
+    int code;
+    amx_error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+namespace detail {
+inline void check(int rc, const char* what)
+{
+    if (rc != AMX_SUCCESS) {
+        char buf[512];
+        amx_last_error(buf, sizeof(buf));
+        throw amx_error(rc, std::string(what) + ": " + buf);
+    }
+}
+template <typename DT, typename GI, typename LI> constexpr void check_types()
+{
+    static_assert(std::is_same<DT, double>::value, "the GPU operator stores FP64 element matrices (DT = double)");
+    static_assert(sizeof(GI) == 8 && std::is_unsigned<GI>::value, "GI must be a 64-bit unsigned type");
+    static_assert(sizeof(LI) == 4 && std::is_unsigned<LI>::value, "LI must be a 32-bit unsigned type");
+}
+}  // namespace detail
+
+template <typename DT, typename GI, typename LI> class Maps {
+  public:
+    Maps(unsigned int rank = 0, unsigned int size = 1)
+    {
+        detail::check_types<DT, GI, LI>();
+        detail::check(amx_maps_create(rank, size, &m_h), "Maps");
+    }
+    ~Maps() { amx_maps_destroy(m_h); }
+    Maps(const Maps&) = delete;
+    Maps& operator=(const Maps&) = delete;
+
+    /* Maps::set_map (include/maps.hpp:398-451) */
+    Error set_map(const LI n_elements_on_rank, const LI* const* element_to_rank_map, const LI* dofs_per_element,
+                  const LI n_all_dofs_on_rank, const GI* rank_to_global_map, const GI owned_global_dof_range_begin,
+                  const GI owned_global_dof_range_end, const GI n_global_dofs, const LI* rank_dof_counts = nullptr)
+    {
+        std::vector<uint32_t> flat;
+        for (LI e = 0; e < n_elements_on_rank; e++)
+            flat.insert(flat.end(), element_to_rank_map[e], element_to_rank_map[e] + dofs_per_element[e]);
+        detail::check(amx_maps_set_map(m_h, n_elements_on_rank, flat.data(), dofs_per_element, n_all_dofs_on_rank,
+                                       reinterpret_cast<const uint64_t*>(rank_to_global_map),
+                                       owned_global_dof_range_begin, owned_global_dof_range_end, n_global_dofs,
+                                       rank_dof_counts),
+                      "Maps::set_map");
+        m_dpe.assign(dofs_per_element, dofs_per_element + n_elements_on_rank);
+        return Error::SUCCESS;
+    }
+
+    /* Maps::set_bdr_map (include/maps.hpp:828-880) */
+    Error set_bdr_map(GI* constrainedDofs, DT* prescribedValues, LI numConstraints)
+    {
+        detail::check(amx_maps_set_bdr_map(m_h, reinterpret_cast<const uint64_t*>(constrainedDofs), prescribedValues,
+                                           numConstraints),
+                      "Maps::set_bdr_map");
+        return Error::SUCCESS;
+    }
+
+    amx_maps_info info() const
+    {
+        amx_maps_info i;
+        detail::check(amx_maps_get_info(m_h, &i), "Maps::info");
+        return i;
+    }
+    LI get_NumElems() const { return info().n_elems; }
+    LI get_NumDofs() const { return info().n_owned; }
+    LI get_NumDofsTotal() const { return info().n_total; }
+    LI get_NumPreGhostDofs() const { return info().n_pre_ghost; }
+    LI get_NumPostGhostDofs() const { return info().n_post_ghost; }
+    LI get_n_owned_constraints() const { return info().n_owned_constraints; }
+    const LI* get_DofsPerElem() const { return m_dpe.data(); }
+    std::vector<LI> get_LocalMapFlat() const
+    {
+        std::vector<LI> out(info().n_entries);
+        detail::check(amx_maps_get_local_map(m_h, out.data()), "Maps::get_LocalMap");
+        return out;
+    }
+    const amx_maps* handle() const { return m_h; }
+
+  private:
+    amx_maps* m_h = nullptr;
+    std::vector<LI> m_dpe;
+};
+
+/* par::aMat CRTP facade (include/aMat.hpp:75-242): forwards to Derived */
+template <typename Derived, typename DT, typename GI, typename LI> class aMat {
+  public:
+    typedef DT DTType;
+    typedef GI GIType;
+    typedef LI LIType;
+    Error set_matfree_type(MATFREE_TYPE t) { return static_cast<Derived*>(this)->set_matfree_type_impl(t); }
+    template <typename MatrixType>
+    Error set_element_matrix(LI eid, const MatrixType& e_mat, LI block_i, LI block_j, LI blocks_dim)
+    {
+        return static_cast<Derived*>(this)->set_element_matrix_impl(eid, e_mat, block_i, block_j, blocks_dim);
+    }
+    Error finalize() { return static_cast<Derived*>(this)->finalize_impl(); }
+    Error matvec(DT* v, const DT* u, bool isGhosted = false)
+    {
+        return static_cast<Derived*>(this)->matvec_impl(v, u, isGhosted);
+    }
+};
+
+template <typename DT, typename GI, typename LI> class aMatFree : public aMat<aMatFree<DT, GI, LI>, DT, GI, LI> {
+  public:
+    /* aMatFree(Maps&, BC_METH) (include/aMatFree.hpp:405-439) */
+    aMatFree(Maps<DT, GI, LI>& mesh_maps, BC_METH bcType = BC_METH::BC_IMATRIX, int device = 0)
+        : m_maps(mesh_maps)
+    {
+        detail::check(amx_op_create(mesh_maps.handle(), (int)bcType, device, &m_h), "aMatFree");
+    }
+    ~aMatFree() { amx_op_destroy(m_h); }
+    aMatFree(const aMatFree&) = delete;
+    aMatFree& operator=(const aMatFree&) = delete;
+
+    Error set_matfree_type_impl(MATFREE_TYPE t)
+    {
+        detail::check(amx_op_set_matfree_type(m_h, (int)t), "set_matfree_type");
+        return Error::SUCCESS;
+    }
+    /* aMatFree::set_num_streams (include/aMatFree.hpp:230-237) */
+    Error set_num_streams(LI nStreams)
+    {
+        detail::check(amx_op_set_num_streams(m_h, nStreams), "set_num_streams");
+        return Error::SUCCESS;
+    }
+    /* aMatFree::set_element_matrix (include/aMatFree.hpp:854-880); MatrixType
+     * needs rows(), cols() and operator()(r, c) (an Eigen matrix works) */
+    template <typename MatrixType>
+    Error set_element_matrix_impl(LI eid, const MatrixType& e_mat, LI block_i, LI block_j, LI blocks_dim)
+    {
+        const LI n = (LI)e_mat.rows();
+        if ((LI)e_mat.cols() != n) throw amx_error(AMX_INVALID_ARGUMENT, "set_element_matrix: block is not square");
+        m_buf.resize((size_t)n * n);
+        for (LI r = 0; r < n; r++)
+            for (LI c = 0; c < n; c++) m_buf[(size_t)r * n + c] = e_mat(r, c);
+        detail::check(amx_op_set_element_matrix(m_h, eid, m_buf.data(), AMX_ROW_MAJOR, block_i, block_j, blocks_dim),
+                      "set_element_matrix");
+        return Error::SUCCESS;
+    }
+    using aMat<aMatFree<DT, GI, LI>, DT, GI, LI>::set_element_matrix;
+    /* multi-block overload (include/aMatFree.hpp:141-146) */
+    template <typename MatrixType>
+    Error set_element_matrix(LI eid, LI* ind_non_zero_block_i, LI* ind_non_zero_block_j,
+                             const MatrixType** non_zero_block_mats, LI num_non_zero_blocks)
+    {
+        for (LI b = 0; b < num_non_zero_blocks; b++)
+            set_element_matrix_impl(eid, *non_zero_block_mats[b], ind_non_zero_block_i[b], ind_non_zero_block_j[b],
+                                    num_non_zero_blocks);
+        return Error::SUCCESS;
+    }
+    Error finalize_impl()
+    {
+        detail::check(amx_op_finalize(m_h), "finalize");
+        return Error::SUCCESS;
+    }
+    /* aMatFree::matvec (include/aMatFree.hpp:1681-1775), host arrays */
+    Error matvec_impl(DT* v, const DT* u, bool isGhosted)
+    {
+        detail::check(amx_op_matvec(m_h, v, u, isGhosted ? 1 : 0), "matvec");
+        return Error::SUCCESS;
+    }
+    /* MatMult_mf semantics (include/aMatFree.hpp:3893-3941), owned-length host arrays */
+    Error matmult_mf(DT* vv, const DT* uu)
+    {
+        detail::check(amx_op_matmult_mf(m_h, vv, uu), "MatMult_mf");
+        return Error::SUCCESS;
+    }
+    /* MatGetDiagonal_mf (include/aMatFree.hpp:3944-3974), owned length */
+    Error get_diagonal(DT* diag)
+    {
+        detail::check(amx_op_get_diagonal(m_h, diag), "get_diagonal");
+        return Error::SUCCESS;
+    }
+    /* apply_bc -> apply_bc_rhs (include/aMatFree.hpp:957-971, 4536-4700), owned-length array */
+    Error apply_bc(DT* rhs)
+    {
+        detail::check(amx_op_apply_bc_rhs(m_h, rhs), "apply_bc");
+        return Error::SUCCESS;
+    }
+    /* device forms: ghosted-length device arrays on a HIP stream */
+    Error matvec_dev(DT* v, const DT* u, void* stream = nullptr)
+    {
+        detail::check(amx_op_matvec_dev(m_h, v, u, stream), "matvec_dev");
+        return Error::SUCCESS;
+    }
+    Error matmult_dev(DT* v, const DT* u, void* stream = nullptr)
+    {
+        detail::check(amx_op_matmult_dev(m_h, v, u, stream), "matmult_dev");
+        return Error::SUCCESS;
+    }
+    amx_op_info info() const
+    {
+        amx_op_info i;
+        detail::check(amx_op_get_info(m_h, &i), "info");
+        return i;
+    }
+    const Maps<DT, GI, LI>& get_maps() { return m_maps; }
+    amx_op* handle() { return m_h; }
+
+#ifdef BUILD_WITH_PETSC
+    /* aMatFree::get_matrix (include/aMatFree.hpp:535-570): a MATSHELL whose
+     * MATOP_MULT / MATOP_GET_DIAGONAL run on the GPU (one rank). */
+    Mat& get_matrix()
+    {
+        MatCreateShell(PETSC_COMM_SELF, m_maps.get_NumDofs(), m_maps.get_NumDofs(), PETSC_DETERMINE, PETSC_DETERMINE,
+                       this, &m_pMat);
+        MatShellSetOperation(m_pMat, MATOP_MULT, (void (*)(void))shell_mult);
+        MatShellSetOperation(m_pMat, MATOP_GET_DIAGONAL, (void (*)(void))shell_diag);
+        return m_pMat;
+    }
+
+  private:
+    static PetscErrorCode shell_mult(Mat A, Vec u, Vec v)
+    {
+        aMatFree* self;
+        MatShellGetContext(A, &self);
+        const PetscScalar* uu;
+        PetscScalar* vv;
+        VecGetArrayRead(u, &uu);
+        VecGetArray(v, &vv);
+        const int rc = amx_op_matmult_mf(self->m_h, vv, uu);
+        VecRestoreArrayRead(u, &uu);
+        VecRestoreArray(v, &vv);
+        return rc == AMX_SUCCESS ? 0 : PETSC_ERR_LIB;
+    }
+    static PetscErrorCode shell_diag(Mat A, Vec d)
+    {
+        aMatFree* self;
+        MatShellGetContext(A, &self);
+        PetscScalar* dd;
+        VecGetArray(d, &dd);
+        const int rc = amx_op_get_diagonal(self->m_h, dd);
+        VecRestoreArray(d, &dd);
+        return rc == AMX_SUCCESS ? 0 : PETSC_ERR_LIB;
+    }
+    Mat m_pMat;
+#endif
+
+  private:
+    Maps<DT, GI, LI>& m_maps;
+    amx_op* m_h = nullptr;
+    std::vector<double> m_buf;
+};
+
+}  // namespace par
+
+#endif /* AMAT_AMD_AMATFREE_HPP */

@@ -1,0 +1,199 @@
+/*
+ * amx_oracle.h -- CPU restatement of aMat's assembly-free matvec path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product (amat_amd/) never links or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the paralab/aMat source tree).  Types follow the reference instantiation
+ * <DT,GI,LI> = <double, unsigned long, unsigned int>
+ * (examples/src/example03a.cpp:423-426).
+ *
+ * Pinning (what checks each function; see tests/test_oracle_*.py):
+ *   orc_build_scatter_map      golden fixtures from the reference maps.hpp compiled
+ *                              and run under MPICH (1..4 ranks); closed-form
+ *                              1-rank identity.
+ *   orc_set_bdr_map            golden fixtures (reference maps.hpp); coordinate
+ *                              predicates on structured meshes.
+ *   orc_identify_independent   golden fixtures (reference maps.hpp).
+ *   orc_matvec_ghosted_omp     closed-form 27-point stencil (hex8 Laplacian);
+ *                              scipy-assembled sparse K @ u; rigid-body null
+ *                              space (hex20 elasticity); row sums (tet10).
+ *   orc_matmult_mf             identity-row semantics checked on an assembled
+ *                              scipy matrix with rows/cols of constrained DoFs
+ *                              replaced by the identity.
+ *   orc_matmult_penalty        independent assembled-CSR path vs the EbE path
+ *                              plus (p - diag) u correction.
+ *   orc_get_diagonal           scipy-assembled diagonal; BC values.
+ *   orc_trace                  numpy trace of the block-diagonal element matrices.
+ *   orc_build_ew_maps /
+ *   orc_gather_ew              equality with the atomic-order matvec at 1 thread.
+ *   orc_pcg                    scipy.sparse.linalg direct solve on tiny systems.
+ */
+#ifndef AMX_ORACLE_H
+#define AMX_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* par::Error values, include/enums.hpp:24-37 */
+enum {
+    ORC_SUCCESS = 0,
+    ORC_INDEX_OUT_OF_BOUNDS = 1,
+    ORC_NULL_L2G_MAP = 4,
+    ORC_GHOST_NODE_NOT_FOUND = 5,
+    ORC_GLOBAL_DOF_ID_NOT_FOUND = 12
+};
+
+/* include/aMat.hpp:68 */
+#define ORC_PENALTY_FACTOR 100.0
+
+/* ---------------------------------------------------------------- maps */
+
+/* Maps::set_map (include/maps.hpp:398-451) + buildScatterMap (:519-826) for
+ * one rank.  The MPI_Allgather of owned-DoF counts (:547) is an input
+ * (rank_dof_counts[size]); the Alltoallv request exchange (:738) is split out:
+ * this call produces the ghost-gid request buffer (sendBuf :723-729, grouped by
+ * owner) and its per-owner counts; orc_set_send_ids consumes what peers sent.
+ * Element maps are flattened row after row (element e occupies
+ * dofs_per_elem[e] entries).
+ * Outputs sized by caller: global_map[sum dpe], local_map[sum dpe],
+ * local2global[n_all_dofs], localid2rank[n_all_dofs], ghost_req_counts[size],
+ * ghost_req_gids[n_all_dofs]. */
+int orc_build_scatter_map(uint32_t rank, uint32_t size, const uint32_t* rank_dof_counts,
+                          uint32_t n_elem, const uint32_t* dofs_per_elem,
+                          const uint32_t* elem_to_rank_map, uint32_t n_all_dofs,
+                          const uint64_t* rank_to_global,
+                          uint64_t owned_begin, uint64_t owned_end, uint64_t n_global,
+                          uint64_t* global_map, uint32_t* local_map,
+                          uint64_t* local2global, uint32_t* localid2rank,
+                          uint32_t* n_pre, uint32_t* n_post,
+                          uint32_t* ghost_req_counts, uint64_t* ghost_req_gids);
+
+/* Second half of buildScatterMap (:741-768): global ids other ranks requested
+ * from me (recvBuf of the Alltoallv, grouped by requesting rank) -> local ids
+ * to send (m_uivSendDofIds). */
+int orc_set_send_ids(uint32_t rank, uint32_t size, const uint32_t* rank_dof_counts,
+                     uint32_t n_pre, uint32_t n_recv, const uint64_t* recv_gids,
+                     uint32_t* send_dof_ids);
+
+/* Maps::set_bdr_map (include/maps.hpp:828-880), literal O(E*n*C) scan.
+ * Outputs: owned_cdofs/owned_cvals (<= n_constraints, input order),
+ * bdr_map[sum dpe] in {0,1}, presval_map[sum dpe], owned_free[...] sorted unique
+ * (capacity sum dpe).  Counts returned through n_owned_c / n_owned_free. */
+int orc_set_bdr_map(uint32_t rank, uint32_t size, const uint32_t* rank_dof_counts,
+                    uint32_t n_elem, const uint32_t* dofs_per_elem, const uint64_t* global_map,
+                    const uint64_t* constrained, const double* values, uint32_t n_constraints,
+                    uint64_t* owned_cdofs, double* owned_cvals, uint32_t* n_owned_c,
+                    uint32_t* bdr_map, double* presval_map,
+                    uint64_t* owned_free, uint32_t* n_owned_free);
+
+/* Maps::identifyIndependentElements (include/maps.hpp:918-949). */
+int orc_identify_independent(uint32_t rank, uint32_t size, const uint32_t* rank_dof_counts,
+                             uint32_t n_elem, const uint32_t* dofs_per_elem,
+                             const uint64_t* global_map,
+                             uint32_t* indep, uint32_t* n_indep, uint32_t* dep, uint32_t* n_dep);
+
+/* ------------------------------------------------------------- matvec */
+
+/* aMatFree::matvec_ghosted_OMP (include/aMatFree.hpp:1777-2238), one element
+ * block per element (blocks_dim = 1).  v = 0 (:1789-1792), then for the
+ * independent elements and then the dependent ones: gather ue (:1858-1862),
+ * ve = K_e ue, scatter with omp atomic (:2004-2008).
+ *   layout 0: K_e row-major, scalar GEMV (:1994-1999) -- the default build;
+ *   layout 1: K_e column-major, VECTORIZED_OPENMP omp-simd AXPY (:1965-1973).
+ * K is the concatenation of the element matrices (element e holds
+ * dofs_per_elem[e]^2 values).  order = indep list then dep list; pass
+ * n_dep = 0 and order = 0..E-1 for one rank.  nthreads <= 0 keeps the OpenMP
+ * default; nthreads = 1 gives the sequential (deterministic) summation order. */
+void orc_matvec_ghosted_omp(uint32_t n_elem, const uint32_t* dofs_per_elem,
+                            const uint32_t* local_map, const double* K, int layout,
+                            const uint32_t* indep, uint32_t n_indep,
+                            const uint32_t* dep, uint32_t n_dep,
+                            const double* u, double* v, uint32_t n_total, int nthreads);
+
+/* Same loop with blocks_dim x blocks_dim blocks per element (the block loops
+ * :1829-2011): element e stores blocks_dim[e]^2 blocks of nb x nb row-major
+ * values (nb = dofs_per_elem[e] / blocks_dim[e]); block_present marks non-null
+ * blocks (absent blocks hold no storage).  Sequential (1 thread). */
+void orc_matvec_blocks(uint32_t n_elem, const uint32_t* dofs_per_elem,
+                       const uint32_t* blocks_dim, const uint8_t* block_present,
+                       const uint32_t* local_map, const double* K,
+                       const double* u, double* v, uint32_t n_total);
+
+/* aMatFree::local_to_ghost / ghost_to_local (include/aMatFree.hpp:824-852). */
+void orc_local_to_ghost(uint32_t n_total, uint32_t n_pre, uint32_t n_owned,
+                        double* g, const double* local);
+void orc_ghost_to_local(uint32_t n_pre, uint32_t n_owned, double* local, const double* g);
+
+/* aMatFree::MatMult_mf (include/aMatFree.hpp:3893-3941): identity-row BCs on
+ * owned-length vectors.  owned_cdofs are global ids (Maps::ownedConstrainedDofs);
+ * dof_scan_rank = m_ulvLocalDofScan[rank]. */
+void orc_matmult_mf(uint32_t n_elem, const uint32_t* dofs_per_elem,
+                    const uint32_t* local_map, const double* K,
+                    uint32_t n_total, uint32_t n_pre, uint32_t n_owned,
+                    const uint64_t* owned_cdofs, uint32_t n_owned_c, uint64_t dof_scan_rank,
+                    const double* uu, double* vv, int nthreads);
+
+/* aMatBased::apply_bc_mat, penalty branch (include/aMatBased.hpp:332-347)
+ * applied to the assembled operator (aMatBased::petsc_set_element_matrix
+ * :212-259 adds every K_e entry): assemble K in CSR from the element matrices,
+ * overwrite the diagonal entry of every constrained row with `penalty`
+ * (INSERT_VALUES), then v = A u.  One rank, local ids = global ids.
+ * bdr_map marks constrained element DoFs (Maps::m_uipBdrMap). */
+void orc_matmult_penalty(uint32_t n_elem, const uint32_t* dofs_per_elem,
+                         const uint32_t* local_map, const double* K,
+                         const uint32_t* bdr_map, double penalty,
+                         uint32_t n_total, const double* u, double* v);
+
+/* aMatFree::finalize_end trace (include/aMatFree.hpp:748-785), element order. */
+double orc_trace(uint32_t n_elem, const uint32_t* dofs_per_elem, const double* K);
+
+/* mat_get_diagonal_ghosted (include/aMatFree.hpp:997-1056) followed by
+ * apply_bc_diagonal (:4309-4337): bc_meth 0 = BC_IMATRIX (1.0), 1 = BC_PENALTY
+ * (PENALTY_FACTOR * trace); pass penalty_override > 0 to use that value
+ * instead.  One rank: row ids are local ids. */
+void orc_get_diagonal(uint32_t n_elem, const uint32_t* dofs_per_elem,
+                      const uint32_t* local_map, const double* K,
+                      const uint32_t* bdr_map, int bc_meth, double penalty_override,
+                      uint32_t n_total, double* diag);
+
+/* aMatGpu::build_ewMaps (include/aMatGpu_original_saved.hpp:374-399) with one
+ * stream holding every element in element order: for each local DoF the list
+ * of positions into the stacked element-vector array ve (element e, row r ->
+ * offset(e) + r), CSR form.  offsets[n_total+1], positions[sum dpe]. */
+void orc_build_ew_maps(uint32_t n_elem, const uint32_t* dofs_per_elem,
+                       const uint32_t* local_map, uint32_t n_total,
+                       uint32_t* offsets, uint32_t* positions);
+
+/* aMatGpu::gather_vHost2v_ew (:568-596): v[d] += sum over its list of ve[pos]. */
+void orc_gather_ew(uint32_t n_total, const uint32_t* offsets, const uint32_t* positions,
+                   const double* ve, double* v);
+
+/* Element vectors ve_e = K_e ue_e stacked in element order (the batched GEMV
+ * that aMatGpu::matvec_v1 runs, include/aMatGpu.hpp:532-566, as K_e u_e). */
+void orc_element_products(uint32_t n_elem, const uint32_t* dofs_per_elem,
+                          const uint32_t* local_map, const double* K,
+                          const double* u, double* ve);
+
+/* ------------------------------------------------------------- solver */
+
+/* Jacobi-preconditioned CG (the KSPCG + PCJACOBI pair of
+ * examples/include/solve.hpp:45-65) on the MatMult_mf operator with the
+ * diagonal of orc_get_diagonal; stops when ||b - A x||_2 <= rtol ||b||_2 on the
+ * recursively updated residual (max_it iterations at most).  x starts at 0.
+ * op = 0: identity-row BC operator (orc_matmult_mf, one rank);
+ * op = 1: penalty operator (orc_matmult_penalty).  Returns iterations. */
+int orc_pcg(int op, uint32_t n_elem, const uint32_t* dofs_per_elem,
+            const uint32_t* local_map, const double* K, const uint32_t* bdr_map,
+            const uint64_t* owned_cdofs, uint32_t n_owned_c, double penalty,
+            uint32_t n, const double* b, double* x, double rtol, int max_it,
+            double* final_relres);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,120 @@
+// dropin_example.cpp -- uses the header-only par::aMatFree drop-in the way
+// examples/src/example03a.cpp uses the reference (hex8 Poisson, identity BCs
+// on the boundary, GPU_PURE), then checks the GPU results against the oracle.
+//
+// Test program: built by amat_amd/build.py, run by tests/test_gpu_dropin.py.
+// Prints one line "dropin <case> maxdiff_matvec=... maxdiff_mf=..." per case
+// and exits non-zero on any mismatch.
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "amat_amd/aMatFree.hpp"
+#include "amx_oracle.h"
+
+struct Mat8 {  // anything with rows(), cols(), operator()(r, c) works
+    double a[64];
+    long rows() const { return 8; }
+    long cols() const { return 8; }
+    double operator()(long r, long c) const { return a[r * 8 + c]; }
+};
+
+static int run_case(unsigned Nex, unsigned Ney, unsigned Nez)
+{
+    using Maps = par::Maps<double, unsigned long, unsigned int>;
+    using AFM = par::aMatFree<double, unsigned long, unsigned int>;
+    const unsigned nelem = Nex * Ney * Nez, nnode = (Nex + 1) * (Ney + 1) * (Nez + 1);
+    const double hx = 1.0 / Nex, hy = 1.0 / Ney, hz = 1.0 / Nez;
+    std::vector<std::vector<unsigned>> map(nelem, std::vector<unsigned>(8));
+    std::vector<unsigned*> mapp(nelem);
+    std::vector<unsigned> dpe(nelem, 8);
+    for (unsigned k = 0; k < Nez; k++)
+        for (unsigned j = 0; j < Ney; j++)
+            for (unsigned i = 0; i < Nex; i++) {
+                const unsigned e = Nex * Ney * k + Nex * j + i;
+                const unsigned n0 = i + j * (Nex + 1) + k * (Nex + 1) * (Ney + 1);
+                const unsigned sx = Nex + 1, sy = (Nex + 1) * (Ney + 1);
+                unsigned c[8] = {n0, n0 + 1, n0 + sx + 1, n0 + sx, n0 + sy, n0 + sy + 1, n0 + sy + sx + 1, n0 + sy + sx};
+                for (int q = 0; q < 8; q++) map[e][q] = c[q];
+                mapp[e] = map[e].data();
+            }
+    std::vector<unsigned long> l2g(nnode);
+    for (unsigned i = 0; i < nnode; i++) l2g[i] = i;
+    std::vector<unsigned long> cons;
+    for (unsigned n = 0; n < nnode; n++) {
+        const unsigned i = n % (Nex + 1), j = (n / (Nex + 1)) % (Ney + 1), k = n / ((Nex + 1) * (Ney + 1));
+        if (i == 0 || j == 0 || k == 0 || i == Nex || j == Ney || k == Nez) cons.push_back(n);
+    }
+    std::vector<double> vals(cons.size(), 0.0);
+
+    Maps maps;
+    maps.set_map(nelem, mapp.data(), dpe.data(), nnode, l2g.data(), 0, nnode - 1, nnode);
+    maps.set_bdr_map(cons.data(), vals.data(), (unsigned)cons.size());
+    AFM A(maps, par::BC_METH::BC_IMATRIX);
+    A.set_matfree_type(par::MATFREE_TYPE::GPU_PURE);
+    Mat8 ke;
+    std::vector<double> Kall((size_t)nelem * 64);
+    for (unsigned e = 0; e < nelem; e++) {
+        double xe[24];
+        for (int q = 0; q < 8; q++) {
+            const unsigned n = map[e][q];
+            xe[3 * q] = (n % (Nex + 1)) * hx;
+            xe[3 * q + 1] = ((n / (Nex + 1)) % (Ney + 1)) * hy;
+            xe[3 * q + 2] = (n / ((Nex + 1) * (Ney + 1))) * hz;
+        }
+        par::detail::check(amx_fem_ke_hex8(xe, 2, ke.a), "ke_hex8");
+        for (int q = 0; q < 64; q++) Kall[(size_t)e * 64 + q] = ke.a[q];
+        A.set_element_matrix(e, ke, 0, 0, 1);
+    }
+    A.finalize();
+
+    std::vector<double> u(nnode), v(nnode), vref(nnode), vm(nnode), vmref(nnode);
+    for (unsigned i = 0; i < nnode; i++) u[i] = std::sin(0.37 * i) + 0.1 * std::cos(1.3 * i);
+    A.matvec(v.data(), u.data(), true);
+    A.matmult_mf(vm.data(), u.data());
+
+    std::vector<unsigned> flat, order(nelem);
+    for (unsigned e = 0; e < nelem; e++) {
+        flat.insert(flat.end(), map[e].begin(), map[e].end());
+        order[e] = e;
+    }
+    orc_matvec_ghosted_omp(nelem, dpe.data(), flat.data(), Kall.data(), 0, order.data(), nelem, nullptr, 0, u.data(),
+                           vref.data(), nnode, 1);
+    std::vector<uint64_t> oc(cons.begin(), cons.end());
+    orc_matmult_mf(nelem, dpe.data(), flat.data(), Kall.data(), nnode, 0, nnode, oc.data(), (uint32_t)oc.size(), 0,
+                   u.data(), vmref.data(), 1);
+    double d1 = 0, d2 = 0;
+    for (unsigned i = 0; i < nnode; i++) {
+        d1 = std::fmax(d1, std::fabs(v[i] - vref[i]));
+        d2 = std::fmax(d2, std::fabs(vm[i] - vmref[i]));
+    }
+    std::printf("dropin hex8_%ux%ux%u maxdiff_matvec=%.3e maxdiff_mf=%.3e\n", Nex, Ney, Nez, d1, d2);
+    return (d1 == 0.0 && d2 == 0.0) ? 0 : 1;
+}
+
+int main()
+{
+    int bad = 0;
+    try {
+        bad |= run_case(4, 5, 6);
+        bad |= run_case(16, 16, 16);
+        // error path: methods 1/2 belong to the reference's CPU loops
+        par::Maps<double, unsigned long, unsigned int> m;
+        unsigned row[8] = {0, 1, 2, 3, 4, 5, 6, 7}, *rows[1] = {row}, d8[1] = {8};
+        unsigned long l2g[8] = {0, 1, 2, 3, 4, 5, 6, 7};
+        m.set_map(1, rows, d8, 8, l2g, 0, 7, 8);
+        par::aMatFree<double, unsigned long, unsigned int> A(m);
+        try {
+            A.set_matfree_type(par::MATFREE_TYPE::QUASI_FREE);
+            std::printf("dropin error-path: missing NOT_IMPLEMENTED\n");
+            bad = 1;
+        } catch (const par::amx_error& e) {
+            std::printf("dropin error-path code=%d\n", e.code);
+            bad |= (e.code != AMX_NOT_IMPLEMENTED);
+        }
+    } catch (const std::exception& e) {
+        std::printf("dropin exception: %s\n", e.what());
+        return 2;
+    }
+    return bad;
+}

@@ -1,0 +1,81 @@
+"""Device solvers: FP64 Jacobi-PCG against a direct solve and the oracle's
+PCG; FP64 iterative refinement with FP32 / BF16 inner solves reaching the
+same 1e-10 relative residual, with their distance to the FP64 solution."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle as orc
+from amat_amd import BCOP, PREC, Maps, aMatFree, mesh, problems
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def direct(p, bcop, pen=None):
+    K = p.element_matrices()
+    E, ne = p.gmap.shape
+    g = p.gmap.astype(np.int64)
+    A = sp.csr_matrix((K.reshape(-1), (np.repeat(g, ne, 1).reshape(-1), np.tile(g, (1, ne)).reshape(-1))),
+                      shape=(p.n_dofs, p.n_dofs)).tolil()
+    c = p.constrained.astype(np.int64)
+    if bcop == BCOP.IDENTITY:
+        A[c, :] = 0
+        A[:, c] = 0
+        A[c, c] = 1.0
+    else:
+        A[c, c] = pen
+    return A.tocsc()
+
+
+@pytest.mark.parametrize("which", ["hex8", "hex20_penalty", "tet10"])
+def test_fp64_cg(torch_gpu, which):
+    torch = torch_gpu
+    if which == "hex8":
+        p, bcop, pen = problems.poisson_hex8(10), BCOP.IDENTITY, None
+    elif which == "hex20_penalty":
+        p, bcop, pen = problems.elasticity_hex20(3), BCOP.PENALTY_DIAG, 1e6
+    else:
+        p, bcop, pen = problems.poisson_tet10(4), BCOP.IDENTITY, None
+    op = p.build_operator(aMatFree, Maps, bcop=bcop, penalty=pen)
+    b = p.rhs if p.rhs is not None else mesh.uniform(5, 5, p.n_dofs, -1, 1)
+    if bcop == BCOP.IDENTITY:
+        b = b.copy()
+        b[p.constrained.astype(np.int64)] = 0.0
+    db = dev(torch, b)
+    dx = torch.zeros_like(db)
+    st = op.cg(db, dx, rtol=1e-10, max_it=5000, stream=torch.cuda.current_stream())
+    x = dx.cpu().numpy()
+    assert st.converged and st.rel_residual <= 1e-10, (st.iterations, st.rel_residual)
+    xd = spla.spsolve(direct(p, bcop, pen), b)
+    assert rel_l2(x, xd) < 1e-7
+    maps = orc.set_bdr_map(0, 1, [p.n_dofs], p.gmap, p.constrained, p.prescribed)
+    xo, it, rr = orc.pcg(0 if bcop == BCOP.IDENTITY else 1, p.gmap, p.element_matrices(), maps["bdr_map"],
+                         p.constrained, pen or 0.0, b, rtol=1e-10)
+    assert abs(it - st.iterations) <= 2
+    assert rel_l2(x, xo) < 1e-7
+
+
+@pytest.mark.parametrize("prec", [PREC.FP32, PREC.BF16])
+@pytest.mark.parametrize("which", ["hex8", "tet10"])
+def test_ir_cg(torch_gpu, prec, which):
+    torch = torch_gpu
+    p = problems.poisson_hex8(12) if which == "hex8" else problems.poisson_tet10(5)
+    op = p.build_operator(aMatFree, Maps, precision=PREC.FP64 | prec)
+    b = (p.rhs if p.rhs is not None else mesh.uniform(6, 6, p.n_dofs, -1, 1)).copy()
+    b[p.constrained.astype(np.int64)] = 0.0
+    db = dev(torch, b)
+    x64 = torch.zeros_like(db)
+    s64 = op.cg(db, x64, rtol=1e-10, stream=torch.cuda.current_stream())
+    xir = torch.zeros_like(db)
+    sir = op.ir_cg(prec, db, xir, rtol=1e-10, inner_rtol=1e-2, stream=torch.cuda.current_stream())
+    err = rel_l2(xir.cpu().numpy(), x64.cpu().numpy())
+    print(f"IR-CG {prec.name} {which}: outer={sir.outer_iterations} inner={sir.iterations} "
+          f"(FP64 CG {s64.iterations}) relres={sir.rel_residual:.2e} err_vs_fp64={err:.2e}")
+    assert sir.converged and sir.rel_residual <= 1e-10
+    assert err < 1e-8
