@@ -88,7 +88,8 @@ struct Op {
         if (host_sc) (void)hipHostFree(host_sc);
     }
     double penalty() const { return penalty_set ? penalty_value : kPenaltyFactor * trace; }
-    hipStream_t pick(void* s) const { return s ? static_cast<hipStream_t>(s) : stream; }
+    // the caller's stream; NULL is the HIP default stream (not the handle's)
+    hipStream_t pick(void* s) const { return static_cast<hipStream_t>(s); }
     bool has(int prec_bit) const { return (prec_mask & prec_bit) != 0; }
 };
 

@@ -15,7 +15,9 @@
  * after the call returns.  Device buffers belong to the handle.
  * Threading: a handle is used by one host thread at a time; distinct handles
  * are independent.  Device entry points (*_dev) are asynchronous on the given
- * HIP stream (NULL = the handle's own stream).
+ * HIP stream; NULL is the HIP default (null) stream, as everywhere in HIP.
+ * amx_op_stream() returns the handle's own stream for callers that want it.
+ * Host-array entry points run on the handle's stream and return synchronized.
  * Errors: every call returns an int status (AMX_SUCCESS = 0); the values below
  * AMX_INVALID_ARGUMENT mirror par::Error (include/enums.hpp:24-37).  The
  * message of the last failure on the calling thread is in amx_last_error().
