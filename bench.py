@@ -62,13 +62,13 @@ def cpu_baseline(p, n_sample_layers: int, reps: int, threads: int) -> dict:
 def load_traffic(name: str):
     """HBM bytes per launch of the element kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None when absent)."""
-    f = ROOT / "profiles" / f"{name}_traffic.json"
+    f = ROOT / "profiles" / f"{name}_traffic.md"
     if not f.exists():
         return None
-    try:
-        return float(json.loads(f.read_text())["bytes_per_launch"])
-    except Exception:
-        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def main() -> int:
@@ -81,6 +81,9 @@ def main() -> int:
     ap.add_argument("--cpu-layers", type=int, default=4, help="element layers in the CPU baseline sample")
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the timed region, copy 1 GiB device-to-device 5 times (known bytes for the "
+                         "FETCH_SIZE / WRITE_SIZE counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU's share; --gpus {args.gpus} is not run here (multi-GPU runs are out of "
@@ -117,11 +120,18 @@ def main() -> int:
     # dominant kernel: mean duration of the element-kernel launches alone
     ms_elem, ms_gather, ms_tot = op.time_kernels_dev(args.steps, v, u, s)
     log(f"[bench] kernel timers: element {ms_elem:.4f} ms, gather {ms_gather:.4f} ms")
+    if args.calibrate_copy:
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     E, n, N = p.n_elem, p.dofs_per_elem, p.n_dofs
     elem_bytes = E * (n * n * 8 + 4 * n) + N * 8       # K + map + u read
     alg_bytes = info.alg_bytes_fp64                     # + v write: the whole matvec
     achieved = elem_bytes / (ms_elem * 1e-3)
-    traffic = load_traffic(f"hex20_{args.n}_fp64_ebe")
+    traffic = load_traffic(f"hex20_{args.n}_fp64_ebe")  # rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated
     out = {
         "metric": METRIC,
         "value": N / (ms * 1e-3) / 1e9,
