@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "amx_internal.hpp"
 #include "amx_kernels.hpp"
@@ -155,7 +156,7 @@ template <int N, typename KT, typename VT, bool MASKED, bool ATOMIC>
 __global__ __launch_bounds__(256) void ebe_tile_kernel(const KT* __restrict__ K, const uint32_t* __restrict__ map,
                                                       const VT* __restrict__ u, VT* __restrict__ out,
                                                       uint64_t ve_base, uint32_t tile_begin, uint32_t n_tiles,
-                                                      const uint32_t* __restrict__ skip)
+                                                      const uint32_t* __restrict__ skip, const float* __restrict__ kdiag)
 {
     using AT = VT;
     if (skip && *skip) return;  // converged solver: nothing to do
@@ -184,6 +185,9 @@ __global__ __launch_bounds__(256) void ebe_tile_kernel(const KT* __restrict__ K,
 #pragma unroll
         for (int p = 0; p < P; ++p) acc = madd(acc, (AT)k.get(p), bcast<G, N>(ue, g, j * P + p));
     }
+    // BF16 storage keeps the diagonal in FP32, corrected for the row's
+    // off-diagonal rounding (row sums preserved); the BF16 diagonal slot is 0
+    if constexpr (std::is_same<KT, uint16_t>::value) acc = madd(acc, (AT)kdiag[row], ue);
     if (lane < (uint32_t)GN) {
         if constexpr (ATOMIC) atomic_add(&out[m & kIdxMask], acc);
         else out[ve_base + row] = acc;
@@ -198,7 +202,8 @@ template <typename KT, typename VT, bool MASKED, bool ATOMIC>
 __global__ __launch_bounds__(256) void ebe_generic_kernel(const KT* __restrict__ K, const uint32_t* __restrict__ map,
                                                          const VT* __restrict__ u, VT* __restrict__ out,
                                                          uint64_t ve_base, uint32_t n, uint32_t slot_begin,
-                                                         uint32_t n_slots, const uint32_t* __restrict__ skip)
+                                                         uint32_t n_slots, const uint32_t* __restrict__ skip,
+                                                         const float* __restrict__ kdiag)
 {
     if (skip && *skip) return;
     __shared__ VT ue[4][kGenericMaxN];
@@ -222,6 +227,7 @@ __global__ __launch_bounds__(256) void ebe_generic_kernel(const KT* __restrict__
             k.load(ke + (uint64_t)c * n + r);
             acc = madd(acc, (VT)k.get(0), ue[w][c]);
         }
+        if constexpr (std::is_same<KT, uint16_t>::value) acc = madd(acc, (VT)kdiag[slot * n + r], ue[w][r]);
         if constexpr (ATOMIC) atomic_add(&out[map[slot * n + r] & kIdxMask], acc);
         else out[ve_base + slot * n + r] = acc;
     }
@@ -290,7 +296,8 @@ __global__ void relayout_rowmajor_kernel(const double* __restrict__ src, const u
     uint32_t r, c;
     ti.decompose(idx, slot, r, c);
     const uint32_t e = slot_elem[slot];
-    const double val = (e == kPadSlot) ? 0.0 : src[(uint64_t)e * ti.N * ti.N + (uint64_t)r * ti.N + c];
+    double val = (e == kPadSlot) ? 0.0 : src[(uint64_t)e * ti.N * ti.N + (uint64_t)r * ti.N + c];
+    if (std::is_same<KT, uint16_t>::value && r == c) val = 0.0;  // diagonal kept in FP32 (bf16_diag_kernel)
     dst[idx] = from_f64<KT>(val);
 }
 
@@ -305,8 +312,33 @@ __global__ void relayout_scaled_kernel(const double* __restrict__ kref, const do
     uint32_t r, c;
     ti.decompose(idx, slot, r, c);
     const uint32_t e = slot_elem[slot];
-    const double val = (e == kPadSlot) ? 0.0 : scale[e] * kref[(uint64_t)r * ti.N + c];
+    double val = (e == kPadSlot) ? 0.0 : scale[e] * kref[(uint64_t)r * ti.N + c];
+    if (std::is_same<KT, uint16_t>::value && r == c) val = 0.0;  // diagonal kept in FP32 (bf16_diag_kernel)
     dst[idx] = from_f64<KT>(val);
+}
+
+// FP32 diagonal of the BF16 storage: d_r = K_rr + sum_{j != r} (K_rj - bf16(K_rj)),
+// so that every row of the stored matrix sums to the FP64 row sum (constants
+// stay in the null space of a Laplacian element); symmetric K stays symmetric.
+__global__ void bf16_diag_kernel(const double* __restrict__ src, const double* __restrict__ kref,
+                                 const double* __restrict__ scale, const uint32_t* __restrict__ slot_elem,
+                                 uint32_t N, uint64_t n_rows, float* __restrict__ out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;  // slot * N + r
+    if (i >= n_rows) return;
+    const uint64_t slot = i / N;
+    const uint32_t r = (uint32_t)(i - slot * N);
+    const uint32_t e = slot_elem[slot];
+    if (e == kPadSlot) {
+        out[i] = 0.0f;
+        return;
+    }
+    double d = 0.0;
+    for (uint32_t j = 0; j < N; j++) {
+        const double kv = src ? src[(uint64_t)e * N * N + (uint64_t)r * N + j] : scale[e] * kref[(uint64_t)r * N + j];
+        d += (j == r) ? kv : kv - (double)bf16_to_f32(f64_to_bf16(kv));
+    }
+    out[i] = (float)d;
 }
 
 __global__ void extract_diag_kernel(const double* __restrict__ K, TiledIndex ti, uint64_t n_rows, double* __restrict__ ve)
@@ -375,11 +407,11 @@ void launch_tile(bool masked, bool atomic, const EbeArgs& a, hipStream_t s)
     const VT* u = static_cast<const VT*>(a.u);
     VT* out = static_cast<VT*>(a.out);
     if (masked) {
-        if (atomic) ebe_tile_kernel<N, KT, VT, true, true><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.tile_begin, a.n_tiles, a.skip);
-        else ebe_tile_kernel<N, KT, VT, true, false><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.tile_begin, a.n_tiles, a.skip);
+        if (atomic) ebe_tile_kernel<N, KT, VT, true, true><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
+        else ebe_tile_kernel<N, KT, VT, true, false><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
     } else {
-        if (atomic) ebe_tile_kernel<N, KT, VT, false, true><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.tile_begin, a.n_tiles, a.skip);
-        else ebe_tile_kernel<N, KT, VT, false, false><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.tile_begin, a.n_tiles, a.skip);
+        if (atomic) ebe_tile_kernel<N, KT, VT, false, true><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
+        else ebe_tile_kernel<N, KT, VT, false, false><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
     }
 }
 
@@ -391,11 +423,11 @@ void launch_generic(uint32_t n, bool masked, bool atomic, const EbeArgs& a, hipS
     const VT* u = static_cast<const VT*>(a.u);
     VT* out = static_cast<VT*>(a.out);
     if (masked) {
-        if (atomic) ebe_generic_kernel<KT, VT, true, true><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, n, a.tile_begin, a.n_tiles, a.skip);
-        else ebe_generic_kernel<KT, VT, true, false><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, n, a.tile_begin, a.n_tiles, a.skip);
+        if (atomic) ebe_generic_kernel<KT, VT, true, true><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, n, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
+        else ebe_generic_kernel<KT, VT, true, false><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, n, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
     } else {
-        if (atomic) ebe_generic_kernel<KT, VT, false, true><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, n, a.tile_begin, a.n_tiles, a.skip);
-        else ebe_generic_kernel<KT, VT, false, false><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, n, a.tile_begin, a.n_tiles, a.skip);
+        if (atomic) ebe_generic_kernel<KT, VT, false, true><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, n, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
+        else ebe_generic_kernel<KT, VT, false, false><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, n, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
     }
 }
 
@@ -473,6 +505,15 @@ void launch_relayout_scaled(const double* k_ref, const double* scale, const uint
     case KType::F32: relayout_scaled_kernel<float><<<nb, 256, 0, s>>>(k_ref, scale, slot_elem, total, ti, (float*)dst); break;
     case KType::BF16: relayout_scaled_kernel<uint16_t><<<nb, 256, 0, s>>>(k_ref, scale, slot_elem, total, ti, (uint16_t*)dst); break;
     }
+    AMX_HIP(hipGetLastError());
+}
+
+void launch_bf16_diag(const double* src_rowmajor, const double* k_ref, const double* scale, const uint32_t* slot_elem,
+                      uint32_t n_slots, uint32_t n, float* out, hipStream_t s)
+{
+    const uint64_t rows = (uint64_t)n_slots * n;
+    if (rows == 0) return;
+    bf16_diag_kernel<<<blocks_for(rows, 256), 256, 0, s>>>(src_rowmajor, k_ref, scale, slot_elem, n, rows, out);
     AMX_HIP(hipGetLastError());
 }
 

@@ -32,6 +32,10 @@ void launch_relayout_from_rowmajor(const double* src, const uint32_t* slot_elem,
 // K_e = scale[elem] * k_ref (row-major n x n) -> tiled layout of kt.
 void launch_relayout_scaled(const double* k_ref, const double* scale, const uint32_t* slot_elem,
                             uint32_t n_slots, const Layout& L, KType kt, void* dst, hipStream_t s);
+// FP32 defect-corrected diagonal of the BF16 storage, [slot][r]; source either
+// row-major matrices (src_rowmajor) or k_ref + scale.
+void launch_bf16_diag(const double* src_rowmajor, const double* k_ref, const double* scale, const uint32_t* slot_elem,
+                      uint32_t n_slots, uint32_t n, float* out, hipStream_t s);
 // diagonal entries K_e(r,r) of every slot row from the tiled FP64 storage into ve.
 void launch_extract_diag(const double* K, const Layout& L, uint32_t n_slots, double* ve, hipStream_t s);
 
@@ -44,6 +48,7 @@ struct EbeArgs {
     uint64_t ve_base;       // first ve row of this group
     uint32_t tile_begin, n_tiles;
     const uint32_t* skip = nullptr;  // device flag: nonzero -> kernel exits (converged solver)
+    const float* kdiag = nullptr;    // BF16 storage: FP32 diagonal [slot][r]
 };
 // vt_double: vectors/accumulation in FP64 (only with KType::F64), else FP32.
 void launch_ebe(const Layout& L, KType kt, bool vt_double, bool masked, bool atomic, const EbeArgs& a,

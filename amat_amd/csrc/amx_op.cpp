@@ -131,6 +131,13 @@ void finalize_op(Op& op)
             else
                 launch_relayout_from_rowmajor(src.as<double>(), dslot.as<uint32_t>(), g.n_slots, g.L[kt], (KType)kt,
                                               g.K[kt].p, s);
+            if (kt == (int)KType::BF16) {
+                g.kdiag16.alloc(std::max<uint64_t>((uint64_t)g.n_slots * n, 1) * sizeof(float));
+                op.device_bytes += g.kdiag16.bytes;
+                launch_bf16_diag(op.scaled ? nullptr : src.as<double>(), op.scaled ? src.as<double>() : nullptr,
+                                 op.scaled ? src_scale.as<double>() : nullptr, dslot.as<uint32_t>(), g.n_slots, n,
+                                 g.kdiag16.as<float>(), s);
+            }
         }
         AMX_HIP(hipStreamSynchronize(s));  // sources are freed at scope exit
     }
@@ -282,6 +289,7 @@ void ebe_launch(Op& op, KType kt, bool vt_double, bool masked, int phase, const 
         a.out = atomic ? v : op.ve.p;
         a.ve_base = g.ve_base;
         a.skip = skip;
+        a.kdiag = kt == KType::BF16 ? g.kdiag16.as<float>() : nullptr;
         if (phase == 0) { a.tile_begin = 0; a.n_tiles = g.tiles_phase0; }
         else if (phase == 1) { a.tile_begin = g.tiles_phase0; a.n_tiles = g.n_tiles - g.tiles_phase0; }
         else { a.tile_begin = 0; a.n_tiles = g.n_tiles; }

@@ -27,6 +27,7 @@ struct Group {
     uint64_t ve_base = 0;             // first ve row
     DevBuf K[3];                      // tiled storage per KType (F64 always)
     DevBuf map;                       // [slot][r] local ids | identity-BC flag
+    DevBuf kdiag16;                   // BF16 storage: FP32 corrected diagonal [slot][r]
 };
 
 struct Op {

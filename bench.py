@@ -71,8 +71,82 @@ def load_traffic(name: str):
     return None
 
 
+def time_matvec(fn, v, u, steps, warmup, torch):
+    s = torch.cuda.current_stream()
+    for _ in range(warmup):
+        fn(v, u, s)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(s)
+    for _ in range(steps):
+        fn(v, u, s)
+    e1.record(s)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def precision_mode(args) -> dict:
+    """Config C3: jittered tet10 Poisson (n^3 hexes x 6 tets), b = M f, identity
+    BCs; FP64 CG vs FP64 iterative refinement with FP32 / BF16 element
+    matrices, all to 1e-10 relative residual, error against the FP64 solution."""
+    import numpy as np
+    import torch
+    from amat_amd import BCOP, PREC, Maps, aMatFree, problems
+    t0 = time.time()
+    p = problems.poisson_tet10(args.n, seed=3, jitter=0.2)
+    log(f"[precision] tet10 {args.n}^3 x 6: {p.n_elem} elements, {p.n_dofs} DoFs ({time.time() - t0:.1f}s)")
+    op = p.build_operator(aMatFree, Maps, precision=PREC.FP64 | PREC.FP32 | PREC.BF16, bcop=BCOP.IDENTITY)
+    del p.K
+    log(f"[precision] operator finalized ({time.time() - t0:.1f}s)")
+    b = op.apply_bc(p.rhs)
+    db = torch.from_numpy(b).cuda()
+    s = torch.cuda.current_stream()
+    res = {"metric": "C3 precision: IR-CG vs FP64 CG to 1e-10", "elements": p.n_elem, "dofs": p.n_dofs}
+    # matvec rates per storage
+    E, n, N = p.n_elem, 10, p.n_dofs
+    u64 = torch.from_numpy(p.random_u()).cuda()
+    v64 = torch.empty_like(u64)
+    u32 = u64.float()
+    v32 = torch.empty_like(u32)
+    for name, fn, v, u, sk, sv in (("fp64", op.matmult_dev, v64, u64, 8, 8),
+                                   ("fp32", lambda a, b_, st: op.matmult_f32_dev(PREC.FP32, a, b_, st), v32, u32, 4, 4),
+                                   ("bf16", lambda a, b_, st: op.matmult_f32_dev(PREC.BF16, a, b_, st), v32, u32, 2, 4)):
+        ms = time_matvec(fn, v, u, args.steps, args.warmup, torch)
+        diag_bytes = E * n * 4 if name == "bf16" else 0
+        alg = E * (n * n * sk + 4 * n) + diag_bytes + N * 2 * sv
+        res[f"matvec_{name}"] = {"ms": ms, "GDoF/s": N / ms / 1e6, "hbm_frac": alg / (ms * 1e-3) / HBM_PEAK,
+                                 "alg_bytes": alg}
+        log(f"[precision] matvec {name}: {ms:.3f} ms, {N / ms / 1e6:.2f} GDoF/s")
+    x64 = torch.zeros_like(db)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    st = op.cg(db, x64, rtol=1e-10, max_it=20000, stream=s)
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t
+    res["cg_fp64"] = {"seconds": t64, "iterations": st.iterations, "rel_residual": st.rel_residual,
+                      "converged": bool(st.converged)}
+    log(f"[precision] FP64 CG: {st.iterations} it, {t64:.3f} s, relres {st.rel_residual:.2e}")
+    xr = x64.cpu().numpy()
+    for prec in (PREC.FP32, PREC.BF16):
+        x = torch.zeros_like(db)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        si = op.ir_cg(prec, db, x, rtol=1e-10, inner_rtol=args.inner_rtol, max_outer=60, max_inner=20000, stream=s)
+        torch.cuda.synchronize()
+        ti = time.perf_counter() - t
+        err = float(np.linalg.norm(x.cpu().numpy() - xr) / np.linalg.norm(xr))
+        res[f"ir_cg_{prec.name.lower()}"] = {"seconds": ti, "outer": si.outer_iterations, "inner": si.iterations,
+                                             "rel_residual": si.rel_residual, "converged": bool(si.converged),
+                                             "rel_error_vs_fp64_solution": err, "speedup_vs_fp64_cg": t64 / ti}
+        log(f"[precision] IR-CG {prec.name}: outer {si.outer_iterations} inner {si.iterations}, {ti:.3f} s, "
+            f"relres {si.rel_residual:.2e}, err vs FP64 {err:.2e}, speedup {t64 / ti:.2f}x")
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
+    ap.add_argument("--mode", choices=["matvec", "precision"], default="matvec")
+    ap.add_argument("--inner-rtol", type=float, default=1e-2)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
@@ -90,11 +164,13 @@ def main() -> int:
               "scope for this build)", file=sys.stderr)
         return 2
 
-    import numpy as np
     import torch
     from amat_amd import BCOP, PREC, SCATTER, Maps, aMatFree, problems
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.mode == "precision":
+        print(json.dumps(precision_mode(args)), flush=True)
+        return 0
     t0 = time.time()
     p = problems.elasticity_hex20(args.n)
     log(f"[bench] mesh hex20 {args.n}^3: {p.n_elem} elements, {p.n_dofs} DoFs ({time.time() - t0:.1f}s)")
