@@ -101,6 +101,14 @@ SIGNATURES = {
     "amx_op_synchronize": (I32, [P]),
     "amx_cg_solve_dev": (I32, [P, P, P, F64, I32, C.POINTER(SolveStats), P]),
     "amx_ir_cg_solve_dev": (I32, [P, I32, P, P, F64, F64, I32, I32, C.POINTER(SolveStats), P]),
+    "amx_mrhs_create": (I32, [P, I32, I32, P]),
+    "amx_mrhs_destroy": (I32, [P]),
+    "amx_mrhs_set_tiles": (I32, [P, I32, I32]),
+    "amx_mrhs_get_tiles": (I32, [P, P, P]),
+    "amx_mrhs_apply_dev": (I32, [P, I32, P, P, P]),
+    "amx_mrhs_autotune": (I32, [P, I32, P, P, I32, P, P]),
+    "amx_mrhs_graph_capture": (I32, [P, I32, P, P, P]),
+    "amx_mrhs_graph_launch": (I32, [P, P]),
     "amx_fem_gauss": (I32, [U32, P]),
     "amx_fem_ke_hex8": (I32, [P, U32, P]),
     "amx_fem_ke_hex20_iso": (I32, [P, F64, F64, U32, P]),

@@ -397,6 +397,59 @@ class aMatFree:
         return st
 
 
+class MultiRHS:
+    """Multi-RHS element products V_i = sum_e P_e^T K_e P_e U_i of several
+    finalized operators in one persistent MFMA launch (FP32 or BF16 storage,
+    FP32 accumulation).  U_i, V_i: float32 device tensors of n_total x nrhs."""
+
+    TILE_SHAPES = [(rt, w) for rt in (32, 64, 128) for w in (1, 2, 4)]
+
+    def __init__(self, ops: Sequence[aMatFree], precision: PREC = PREC.FP32):
+        arr = (C.c_void_p * len(ops))(*[o.handle.value for o in ops])
+        h = C.c_void_p()
+        call("amx_mrhs_create", arr, len(ops), int(precision), C.byref(h))
+        self._h = h
+        self.ops = list(ops)  # keeps the operators alive
+        self.precision = PREC(precision)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _capi.lib().amx_mrhs_destroy(h)
+            self._h = None
+
+    @staticmethod
+    def _lists(U, V):
+        assert len(U) == len(V)
+        return (C.c_void_p * len(U))(*[_ptr(u) for u in U]), (C.c_void_p * len(V))(*[_ptr(v) for v in V])
+
+    def set_tiles(self, rhs_tile: int, waves_per_block: int):
+        call("amx_mrhs_set_tiles", self._h, int(rhs_tile), int(waves_per_block))
+
+    def tiles(self) -> tuple[int, int]:
+        a, b = C.c_int32(), C.c_int32()
+        call("amx_mrhs_get_tiles", self._h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def apply_dev(self, nrhs: int, U, V, stream=None):
+        pu, pv = self._lists(U, V)
+        call("amx_mrhs_apply_dev", self._h, int(nrhs), pu, pv, _stream_ptr(stream))
+
+    def autotune(self, nrhs: int, U, V, reps: int = 5, stream=None) -> dict:
+        """Times every tile shape and keeps the fastest; returns {(rhs_tile, waves): ms}."""
+        pu, pv = self._lists(U, V)
+        res = (C.c_double * 9)()
+        call("amx_mrhs_autotune", self._h, int(nrhs), pu, pv, int(reps), res, _stream_ptr(stream))
+        return {shape: res[i] for i, shape in enumerate(self.TILE_SHAPES) if res[i] < 1e29}
+
+    def graph_capture(self, nrhs: int, U, V, stream):
+        pu, pv = self._lists(U, V)
+        call("amx_mrhs_graph_capture", self._h, int(nrhs), pu, pv, _stream_ptr(stream))
+
+    def graph_launch(self, stream):
+        call("amx_mrhs_graph_launch", self._h, _stream_ptr(stream))
+
+
 def device_count() -> int:
     return int(_capi.lib().amx_device_count())
 
@@ -405,8 +458,8 @@ def version() -> str:
     return _capi.lib().amx_version().decode()
 
 
-__all__ = ["AmatError", "BC_METH", "BCOP", "Error", "MATFREE_TYPE", "Maps", "PREC", "SCATTER", "aMatFree",
-           "device_count", "version"]
+__all__ = ["AmatError", "BC_METH", "BCOP", "Error", "MATFREE_TYPE", "Maps", "MultiRHS", "PREC", "SCATTER",
+           "aMatFree", "device_count", "version"]
 
 
 def flatten_rows(rows: Sequence) -> tuple[np.ndarray, np.ndarray]:

@@ -35,6 +35,7 @@ SOURCES = [
     "amx_solver.cpp",
     "amx_kernels.hip",
     "amx_solver_kernels.hip",
+    "amx_mrhs.hip",
 ]
 
 COMMON = [

@@ -314,6 +314,39 @@ int amx_ir_cg_solve_dev(amx_op* op, int precision, const double* b, double* x, d
                         double inner_rtol, int max_outer, int max_inner,
                         amx_solve_stats* stats, void* stream);
 
+/* ------------------------------------------------ multi-RHS element products
+ * V_i = sum_e P_e^T K_e P_e U_i for several operators i in ONE persistent
+ * launch, each U_i / V_i a device array of n_total x nrhs floats (DoF-major,
+ * the nrhs values of a DoF contiguous).  The reference applies one vector at a
+ * time (aMatFree::matvec, include/aMatFree.hpp:1681-1775; its GPU methods call
+ * a batched GEMV per colour, include/aMatGpu.hpp:21-591); this is the same
+ * product for nrhs vectors at once, on MFMA: FP32 storage with FP32 MFMA, or
+ * BF16 storage and BF16-rounded U with FP32 accumulation.  No boundary
+ * conditions (the plain element sum of aMatFree::matvec); the element rows are
+ * added with FP32 atomics, so the summation order is not fixed.
+ * Element types: 8, 10, 20 and 60 DoFs per element, one type per operator. */
+typedef struct amx_mrhs amx_mrhs;
+/* Copies the operators' element matrices (from their FP64 storage) into the
+ * MFMA layout of precision (AMX_PREC_FP32 or AMX_PREC_BF16).  The operators
+ * must be finalized, on one device; at most 8. */
+int amx_mrhs_create(amx_op* const* ops, int n_ops, int precision, amx_mrhs** out);
+int amx_mrhs_destroy(amx_mrhs* m);
+/* Tile shape: rhs_tile = 32, 64 or 128 RHS columns per wave; waves_per_block
+ * = 1, 2 or 4. */
+int amx_mrhs_set_tiles(amx_mrhs* m, int rhs_tile, int waves_per_block);
+int amx_mrhs_get_tiles(const amx_mrhs* m, int* rhs_tile, int* waves_per_block);
+/* U[i], V[i]: device arrays of operator i (n_ops of each).  V is overwritten. */
+int amx_mrhs_apply_dev(amx_mrhs* m, int nrhs, const float* const* U, float* const* V, void* stream);
+/* Times every tile shape (reps launches each) on the given arrays, keeps the
+ * fastest; results[9] (may be NULL) receives the mean ms per launch in the
+ * order rhs_tile 32/64/128 x waves 1/2/4 (1e30 where a shape was skipped). */
+int amx_mrhs_autotune(amx_mrhs* m, int nrhs, const float* const* U, float* const* V, int reps,
+                      double* results, void* stream);
+/* Captures one apply (zero fill + persistent launch) into a HIP graph bound to
+ * these arrays; stream must not be the default stream. */
+int amx_mrhs_graph_capture(amx_mrhs* m, int nrhs, const float* const* U, float* const* V, void* stream);
+int amx_mrhs_graph_launch(amx_mrhs* m, void* stream);
+
 /* ------------------------------------------------------- element kernels
  * Element matrices of the examples (examples/src/ke_matrix.cpp,
  * examples/src/fe_vector.cpp), FP64 on the host.  Row-major output. */
