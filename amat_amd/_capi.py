@@ -103,8 +103,8 @@ SIGNATURES = {
     "amx_ir_cg_solve_dev": (I32, [P, I32, P, P, F64, F64, I32, I32, C.POINTER(SolveStats), P]),
     "amx_mrhs_create": (I32, [P, I32, I32, P]),
     "amx_mrhs_destroy": (I32, [P]),
-    "amx_mrhs_set_tiles": (I32, [P, I32, I32]),
-    "amx_mrhs_get_tiles": (I32, [P, P, P]),
+    "amx_mrhs_set_tiles": (I32, [P, I32, I32, I32]),
+    "amx_mrhs_get_tiles": (I32, [P, P, P, P]),
     "amx_mrhs_apply_dev": (I32, [P, I32, P, P, P]),
     "amx_mrhs_autotune": (I32, [P, I32, P, P, I32, P, P]),
     "amx_mrhs_graph_capture": (I32, [P, I32, P, P, P]),

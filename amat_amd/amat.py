@@ -402,7 +402,7 @@ class MultiRHS:
     finalized operators in one persistent MFMA launch (FP32 or BF16 storage,
     FP32 accumulation).  U_i, V_i: float32 device tensors of n_total x nrhs."""
 
-    TILE_SHAPES = [(rt, w) for rt in (32, 64, 128) for w in (1, 2, 4)]
+    TILE_SHAPES = [(rt, w, c) for rt in (32, 64, 128) for w in (1, 2, 4) for c in (0, 1)]
 
     def __init__(self, ops: Sequence[aMatFree], precision: PREC = PREC.FP32):
         arr = (C.c_void_p * len(ops))(*[o.handle.value for o in ops])
@@ -415,7 +415,10 @@ class MultiRHS:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            _capi.lib().amx_mrhs_destroy(h)
+            try:
+                _capi.lib().amx_mrhs_destroy(h)
+            except Exception:  # interpreter shutdown
+                pass
             self._h = None
 
     @staticmethod
@@ -423,22 +426,22 @@ class MultiRHS:
         assert len(U) == len(V)
         return (C.c_void_p * len(U))(*[_ptr(u) for u in U]), (C.c_void_p * len(V))(*[_ptr(v) for v in V])
 
-    def set_tiles(self, rhs_tile: int, waves_per_block: int):
-        call("amx_mrhs_set_tiles", self._h, int(rhs_tile), int(waves_per_block))
+    def set_tiles(self, rhs_tile: int, waves_per_block: int, work_split: int = 1):
+        call("amx_mrhs_set_tiles", self._h, int(rhs_tile), int(waves_per_block), int(work_split))
 
-    def tiles(self) -> tuple[int, int]:
-        a, b = C.c_int32(), C.c_int32()
-        call("amx_mrhs_get_tiles", self._h, C.byref(a), C.byref(b))
-        return a.value, b.value
+    def tiles(self) -> tuple[int, int, int]:
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        call("amx_mrhs_get_tiles", self._h, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
 
     def apply_dev(self, nrhs: int, U, V, stream=None):
         pu, pv = self._lists(U, V)
         call("amx_mrhs_apply_dev", self._h, int(nrhs), pu, pv, _stream_ptr(stream))
 
     def autotune(self, nrhs: int, U, V, reps: int = 5, stream=None) -> dict:
-        """Times every tile shape and keeps the fastest; returns {(rhs_tile, waves): ms}."""
+        """Times every launch shape and keeps the fastest; returns {(rhs_tile, waves, split): ms}."""
         pu, pv = self._lists(U, V)
-        res = (C.c_double * 9)()
+        res = (C.c_double * len(self.TILE_SHAPES))()
         call("amx_mrhs_autotune", self._h, int(nrhs), pu, pv, int(reps), res, _stream_ptr(stream))
         return {shape: res[i] for i, shape in enumerate(self.TILE_SHAPES) if res[i] < 1e29}
 

@@ -143,9 +143,88 @@ def precision_mode(args) -> dict:
     return res
 
 
+MFMA_PEAK = {"fp32": 157.3e12, "bf16": 2.5e15}  # MI355X dense MFMA, flop/s
+
+
+def batch_mode(args) -> dict:
+    """Config C4: multi-RHS products of hex20 elasticity 48^3 and hex8 Poisson
+    80x40x20 together in one persistent MFMA launch, nrhs in {8, 32, 128},
+    FP32 and BF16 storage, FP32 vectors (n_total x nrhs, DoF-major).  Per case:
+    autotuned tile shape, mean ms per launch (eager and HIP-graph replay),
+    algorithmic bytes and flops, and the error of RHS column 0 against the FP64
+    operator."""
+    import numpy as np
+    import torch
+    from amat_amd import PREC, Maps, MultiRHS, aMatFree, mesh, problems
+    t0 = time.time()
+    probs = [problems.elasticity_hex20(args.batch_n), problems.poisson_hex8(80, 40, 20)]
+    ops = [p.build_operator(aMatFree, Maps, precision=PREC.FP64) for p in probs]
+    log(f"[batch] operators: " + ", ".join(f"{p.name} E={p.n_elem} N={p.n_dofs}" for p in probs) +
+        f" ({time.time() - t0:.1f}s)")
+    s = torch.cuda.Stream()
+    res = {"metric": "C4 multi-RHS EbE products (hex20 48^3 + hex8 80x40x20, one persistent launch)",
+           "operators": [{"name": p.name, "elements": p.n_elem, "dofs": p.n_dofs, "dofs_per_element": p.dofs_per_elem}
+                         for p in probs], "cases": []}
+    for prec_name in args.batch_precisions.split(","):
+        prec = PREC.FP32 if prec_name == "fp32" else PREC.BF16
+        mr = MultiRHS(ops, prec)
+        sk = 4 if prec == PREC.FP32 else 2
+        for nrhs in [int(x) for x in args.batch_nrhs.split(",")]:
+            U = [torch.from_numpy(mesh.uniform(21 + i, 7, p.n_dofs * nrhs, -1.0, 1.0).astype(np.float32)
+                                  .reshape(p.n_dofs, nrhs)).cuda() for i, p in enumerate(probs)]
+            V = [torch.empty_like(u) for u in U]
+            with torch.cuda.stream(s):
+                tunes = mr.autotune(nrhs, U, V, reps=5, stream=s)
+                for _ in range(args.warmup):
+                    mr.apply_dev(nrhs, U, V, s)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                s.synchronize()
+                e0.record(s)
+                for _ in range(args.steps):
+                    mr.apply_dev(nrhs, U, V, s)
+                e1.record(s)
+                s.synchronize()
+                ms = e0.elapsed_time(e1) / args.steps
+                mr.graph_capture(nrhs, U, V, s)
+                mr.graph_launch(s)
+                s.synchronize()
+                e0.record(s)
+                for _ in range(args.steps):
+                    mr.graph_launch(s)
+                e1.record(s)
+                s.synchronize()
+                ms_graph = e0.elapsed_time(e1) / args.steps
+            # column 0 against the FP64 operator (plain element sum)
+            errs = []
+            for i, (p, op) in enumerate(zip(probs, ops)):
+                u0 = U[i][:, 0].double().contiguous()
+                v0 = torch.empty_like(u0)
+                op.matvec_dev(v0, u0, torch.cuda.current_stream())
+                errs.append(float(torch.linalg.norm(V[i][:, 0].double() - v0) / torch.linalg.norm(v0)))
+            alg = sum(p.n_elem * (p.dofs_per_elem ** 2 * sk + 4 * p.dofs_per_elem) + p.n_dofs * 8 * nrhs
+                      for p in probs)
+            flops = sum(2 * p.n_elem * p.dofs_per_elem ** 2 * nrhs for p in probs)
+            best = min(ms, ms_graph)
+            case = {"precision": prec_name, "nrhs": nrhs, "tile": list(mr.tiles()), "ms": ms, "ms_graph": ms_graph,
+                    "GDoF_rhs_per_s": sum(p.n_dofs for p in probs) * nrhs / (best * 1e-3) / 1e9,
+                    "alg_bytes": alg, "hbm_frac": alg / (best * 1e-3) / HBM_PEAK,
+                    "tflops": flops / (best * 1e-3) / 1e12, "mfma_frac": flops / (best * 1e-3) / MFMA_PEAK[prec_name],
+                    "col0_rel_err_vs_fp64": errs,
+                    "autotune_ms": {f"{rt}x{w}x{c}": round(t, 4) for (rt, w, c), t in tunes.items()}}
+            res["cases"].append(case)
+            log(f"[batch] {prec_name} nrhs {nrhs}: tile {mr.tiles()} {ms:.4f} ms (graph {ms_graph:.4f}), "
+                f"{case['hbm_frac'] * 100:.1f}% HBM, {case['tflops']:.1f} TFLOP/s, err {errs}")
+            del U, V
+        del mr
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--mode", choices=["matvec", "precision"], default="matvec")
+    ap.add_argument("--mode", choices=["matvec", "precision", "batch"], default="matvec")
+    ap.add_argument("--batch-n", type=int, default=48, help="hex20 elements per edge in --mode batch")
+    ap.add_argument("--batch-nrhs", default="8,32,128")
+    ap.add_argument("--batch-precisions", default="fp32,bf16")
     ap.add_argument("--inner-rtol", type=float, default=1e-2)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
@@ -170,6 +249,9 @@ def main() -> int:
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     if args.mode == "precision":
         print(json.dumps(precision_mode(args)), flush=True)
+        return 0
+    if args.mode == "batch":
+        print(json.dumps(batch_mode(args)), flush=True)
         return 0
     t0 = time.time()
     p = problems.elasticity_hex20(args.n)

@@ -331,15 +331,17 @@ typedef struct amx_mrhs amx_mrhs;
  * must be finalized, on one device; at most 8. */
 int amx_mrhs_create(amx_op* const* ops, int n_ops, int precision, amx_mrhs** out);
 int amx_mrhs_destroy(amx_mrhs* m);
-/* Tile shape: rhs_tile = 32, 64 or 128 RHS columns per wave; waves_per_block
- * = 1, 2 or 4. */
-int amx_mrhs_set_tiles(amx_mrhs* m, int rhs_tile, int waves_per_block);
-int amx_mrhs_get_tiles(const amx_mrhs* m, int* rhs_tile, int* waves_per_block);
+/* Launch shape: rhs_tile = 32, 64 or 128 RHS columns per wave;
+ * waves_per_block = 1, 2 or 4; work_split = 1: every wave takes a contiguous
+ * run of element tiles of its XCD's share, 0: the waves interleave over it. */
+int amx_mrhs_set_tiles(amx_mrhs* m, int rhs_tile, int waves_per_block, int work_split);
+int amx_mrhs_get_tiles(const amx_mrhs* m, int* rhs_tile, int* waves_per_block, int* work_split);
 /* U[i], V[i]: device arrays of operator i (n_ops of each).  V is overwritten. */
 int amx_mrhs_apply_dev(amx_mrhs* m, int nrhs, const float* const* U, float* const* V, void* stream);
-/* Times every tile shape (reps launches each) on the given arrays, keeps the
- * fastest; results[9] (may be NULL) receives the mean ms per launch in the
- * order rhs_tile 32/64/128 x waves 1/2/4 (1e30 where a shape was skipped). */
+/* Times every launch shape (reps launches each) on the given arrays, keeps
+ * the fastest; results[18] (may be NULL) receives the mean ms per launch in
+ * the order rhs_tile 32/64/128 x waves 1/2/4 x work_split 0/1 (1e30 where a
+ * shape was skipped). */
 int amx_mrhs_autotune(amx_mrhs* m, int nrhs, const float* const* U, float* const* V, int reps,
                       double* results, void* stream);
 /* Captures one apply (zero fill + persistent launch) into a HIP graph bound to

@@ -91,15 +91,15 @@ def test_single_operator(torch_gpu, case, precision, nrhs):
 
 @pytest.mark.parametrize("precision", [PREC.FP32, PREC.BF16])
 def test_tile_shapes_agree(torch_gpu, precision):
-    """Every (rhs_tile, waves_per_block) gives the same product within the bar."""
+    """Every (rhs_tile, waves_per_block, work_split) gives the same product within the bar."""
     torch = torch_gpu
     probs = [CASES["hex20_elast"](), CASES["hex8"]()]
     ops, mr = make(torch, probs, precision)
     nrhs = 70
     U, dU, dV = inputs(torch, probs, nrhs)
-    for rt, w in MultiRHS.TILE_SHAPES:
-        mr.set_tiles(rt, w)
-        assert mr.tiles() == (rt, w)
+    for rt, w, c in MultiRHS.TILE_SHAPES:
+        mr.set_tiles(rt, w, c)
+        assert mr.tiles() == (rt, w, c)
         for v in dV:
             v.fill_(float("nan"))
         mr.apply_dev(nrhs, dU, dV, torch.cuda.current_stream())
@@ -169,5 +169,7 @@ def test_errors(torch_gpu):
     mr = MultiRHS([op], PREC.FP32)
     with pytest.raises(AmatError):
         mr.set_tiles(48, 1)
+    with pytest.raises(AmatError):
+        mr.set_tiles(32, 1, 2)
     with pytest.raises(AmatError):
         mr.graph_launch(None)
