@@ -31,6 +31,11 @@ class MapsInfo(C.Structure):
         ("owned_begin", C.c_uint64), ("owned_end", C.c_uint64), ("n_global", C.c_uint64)]
 
 
+class ServerInfo(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("active_jobs", "held_jobs", "free_slots", "pad")] + [
+        (n, C.c_uint64) for n in ("workspace_bytes", "steps", "iterations", "completed")]
+
+
 class OpInfo(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("n_elems", "n_total", "n_owned", "n_pre_ghost", "n_groups", "finalized")] + [
         (n, C.c_int32) for n in ("matfree_type", "bc_meth", "bc_operator", "scatter_mode", "precision_mask")] + [
@@ -109,6 +114,14 @@ SIGNATURES = {
     "amx_mrhs_autotune": (I32, [P, I32, P, P, I32, P, P]),
     "amx_mrhs_graph_capture": (I32, [P, I32, P, P, P]),
     "amx_mrhs_graph_launch": (I32, [P, P]),
+    "amx_server_create": (I32, [I32, I32, P]),
+    "amx_server_destroy": (I32, [P]),
+    "amx_server_job_bytes": (I32, [P, I32, P]),
+    "amx_server_submit": (I32, [P, P, I32, P, F64, F64, I32, I32, P]),
+    "amx_server_step": (I32, [P, I32, P]),
+    "amx_server_poll": (I32, [P, P, I32, P]),
+    "amx_server_result": (I32, [P, C.c_int64, P, P]),
+    "amx_server_get_info": (I32, [P, P]),
     "amx_fem_gauss": (I32, [U32, P]),
     "amx_fem_ke_hex8": (I32, [P, U32, P]),
     "amx_fem_ke_hex20_iso": (I32, [P, F64, F64, U32, P]),

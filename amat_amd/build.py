@@ -33,6 +33,7 @@ SOURCES = [
     "amx_fem.cpp",
     "amx_op.cpp",
     "amx_solver.cpp",
+    "amx_serve.cpp",
     "amx_kernels.hip",
     "amx_solver_kernels.hip",
     "amx_mrhs.hip",

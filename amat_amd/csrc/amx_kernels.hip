@@ -152,22 +152,18 @@ template <int G, int N, typename AT> __device__ __forceinline__ AT bcast(AT x, u
 template <typename VT> __device__ __forceinline__ void atomic_add(VT* p, VT x) { unsafeAtomicAdd(p, x); }
 
 // ---------------------------------------------------------------- EbE tile
+// One wave, tile t: G elements x N rows (lane = g*N + r).
 template <int N, typename KT, typename VT, bool MASKED, bool ATOMIC>
-__global__ __launch_bounds__(256) void ebe_tile_kernel(const KT* __restrict__ K, const uint32_t* __restrict__ map,
-                                                      const VT* __restrict__ u, VT* __restrict__ out,
-                                                      uint64_t ve_base, uint32_t tile_begin, uint32_t n_tiles,
-                                                      const uint32_t* __restrict__ skip, const float* __restrict__ kdiag)
+__device__ __forceinline__ void ebe_tile_body(const KT* __restrict__ K, const uint32_t* __restrict__ map,
+                                              const VT* __restrict__ u, VT* __restrict__ out, uint64_t ve_base,
+                                              uint32_t t, const float* __restrict__ kdiag)
 {
     using AT = VT;
-    if (skip && *skip) return;  // converged solver: nothing to do
     constexpr int G = 64 / N;
     constexpr int GN = G * N;
     constexpr int P = pack_of(N, KSize<KT>::v);
     constexpr int NP = N / P;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t tloc = xcd_block(blockIdx.x, gridDim.x) * 4u + (threadIdx.x >> 6);
-    if (tloc >= n_tiles) return;  // uniform per wave
-    const uint32_t t = tile_begin + tloc;
     // lanes past G*N mirror the last active lane (same addresses, no traffic)
     const uint32_t lc = lane < (uint32_t)GN ? lane : (uint32_t)GN - 1u;
     const uint32_t g = lc / N;
@@ -192,6 +188,51 @@ __global__ __launch_bounds__(256) void ebe_tile_kernel(const KT* __restrict__ K,
         if constexpr (ATOMIC) atomic_add(&out[m & kIdxMask], acc);
         else out[ve_base + row] = acc;
     }
+}
+
+template <int N, typename KT, typename VT, bool MASKED, bool ATOMIC>
+__global__ __launch_bounds__(256) void ebe_tile_kernel(const KT* __restrict__ K, const uint32_t* __restrict__ map,
+                                                      const VT* __restrict__ u, VT* __restrict__ out,
+                                                      uint64_t ve_base, uint32_t tile_begin, uint32_t n_tiles,
+                                                      const uint32_t* __restrict__ skip, const float* __restrict__ kdiag)
+{
+    if (skip && *skip) return;  // converged solver: nothing to do
+    const uint32_t tloc = xcd_block(blockIdx.x, gridDim.x) * 4u + (threadIdx.x >> 6);
+    if (tloc >= n_tiles) return;  // uniform per wave
+    ebe_tile_body<N, KT, VT, MASKED, ATOMIC>(K, map, u, out, ve_base, tile_begin + tloc, kdiag);
+}
+
+// index of the entry that owns work item x: the largest j < n with prefix[j] <= x
+// (x wave-uniform, so the search runs on the scalar unit)
+__device__ __forceinline__ uint32_t owner_of(const uint32_t* __restrict__ prefix, uint32_t n, uint32_t x)
+{
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Many operators (jobs) in one launch: wave -> tile of entry owner_of(prefix).
+// Same per-tile arithmetic as ebe_tile_kernel (deterministic scatter only).
+template <int N, typename KT, typename VT>
+__global__ __launch_bounds__(256) void ebe_tile_batched_kernel(const EbeJob* __restrict__ jobs,
+                                                              const uint32_t* __restrict__ prefix, uint32_t n_jobs)
+{
+    const uint32_t gt = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4u + (threadIdx.x >> 6));
+    if (gt >= prefix[n_jobs]) return;
+    const uint32_t j = owner_of(prefix, n_jobs, gt);
+    const EbeJob& jb = jobs[j];
+    if (jb.skip && *jb.skip) return;
+    const uint32_t t = jb.tile_begin + (gt - prefix[j]);
+    if (jb.masked)
+        ebe_tile_body<N, KT, VT, true, false>(static_cast<const KT*>(jb.K), jb.map, static_cast<const VT*>(jb.u),
+                                              static_cast<VT*>(jb.out), jb.ve_base, t, jb.kdiag);
+    else
+        ebe_tile_body<N, KT, VT, false, false>(static_cast<const KT*>(jb.K), jb.map, static_cast<const VT*>(jb.u),
+                                               static_cast<VT*>(jb.out), jb.ve_base, t, jb.kdiag);
 }
 
 // Any n: one wave per element, column-major storage (G = 1, P = 1), element
@@ -235,14 +276,11 @@ __global__ __launch_bounds__(256) void ebe_generic_kernel(const KT* __restrict__
 
 // ------------------------------------------------------------------ gather
 template <typename VT, int BC>
-__global__ __launch_bounds__(256) void gather_kernel(const uint32_t* __restrict__ off, const uint32_t* __restrict__ pos,
-                                                    const VT* __restrict__ ve, const VT* __restrict__ u,
-                                                    const uint8_t* __restrict__ flag, const double* __restrict__ coef,
-                                                    VT* __restrict__ v, uint32_t n, const uint32_t* __restrict__ skip)
+__device__ __forceinline__ void gather_body(const uint32_t* __restrict__ off, const uint32_t* __restrict__ pos,
+                                            const VT* __restrict__ ve, const VT* __restrict__ u,
+                                            const uint8_t* __restrict__ flag, const double* __restrict__ coef,
+                                            VT* __restrict__ v, uint32_t d)
 {
-    if (skip && *skip) return;
-    const uint32_t d = xcd_block(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
-    if (d >= n) return;
     const uint32_t b = off[d], e = off[d + 1];
     VT s = VT(0);
     for (uint32_t q = b; q < e; ++q) s += ve[pos[q]];
@@ -252,6 +290,38 @@ __global__ __launch_bounds__(256) void gather_kernel(const uint32_t* __restrict_
         if (flag[d]) s = (VT)((double)s + coef[d] * (double)u[d]);  // penalty row in FP64
     }
     v[d] = s;
+}
+
+template <typename VT, int BC>
+__global__ __launch_bounds__(256) void gather_kernel(const uint32_t* __restrict__ off, const uint32_t* __restrict__ pos,
+                                                    const VT* __restrict__ ve, const VT* __restrict__ u,
+                                                    const uint8_t* __restrict__ flag, const double* __restrict__ coef,
+                                                    VT* __restrict__ v, uint32_t n, const uint32_t* __restrict__ skip)
+{
+    if (skip && *skip) return;
+    const uint32_t d = xcd_block(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
+    if (d >= n) return;
+    gather_body<VT, BC>(off, pos, ve, u, flag, coef, v, d);
+}
+
+// many jobs in one launch: block -> 256 DoFs of entry owner_of(prefix)
+template <typename VT>
+__global__ __launch_bounds__(256) void gather_batched_kernel(const GatherJob* __restrict__ jobs,
+                                                            const uint32_t* __restrict__ prefix, uint32_t n_jobs)
+{
+    const uint32_t gb = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x));
+    if (gb >= prefix[n_jobs]) return;
+    const uint32_t j = owner_of(prefix, n_jobs, gb);
+    const GatherJob& jb = jobs[j];
+    if (jb.skip && *jb.skip) return;
+    const uint32_t d = (gb - prefix[j]) * 256u + threadIdx.x;
+    if (d >= jb.n) return;
+    const VT* ve = static_cast<const VT*>(jb.ve);
+    const VT* u = static_cast<const VT*>(jb.u);
+    VT* v = static_cast<VT*>(jb.v);
+    if (jb.bc == 1) gather_body<VT, 1>(jb.off, jb.pos, ve, u, jb.flag, jb.coef, v, d);
+    else if (jb.bc == 2) gather_body<VT, 2>(jb.off, jb.pos, ve, u, jb.flag, jb.coef, v, d);
+    else gather_body<VT, 0>(jb.off, jb.pos, ve, u, jb.flag, jb.coef, v, d);
 }
 
 template <typename VT, int BC>
@@ -560,6 +630,48 @@ void launch_gather(bool vt_double, int bc, const uint32_t* off, const uint32_t* 
 #undef AMX_GATHER
     AMX_HIP(hipGetLastError());
 }
+
+void launch_ebe_batched(uint32_t n, KType kt, bool vt_double, const EbeJob* jobs, const uint32_t* prefix,
+                        uint32_t n_jobs, uint32_t total_tiles, hipStream_t s)
+{
+    if (total_tiles == 0 || n_jobs == 0) return;
+    const uint32_t nb = blocks_for(total_tiles, 4);
+#define AMX_EBE_B(NN, KTT, VTT) ebe_tile_batched_kernel<NN, KTT, VTT><<<nb, 256, 0, s>>>(jobs, prefix, n_jobs)
+#define AMX_EBE_N(KTT, VTT)                                                                                        \
+    switch (n) {                                                                                                   \
+    case 4: AMX_EBE_B(4, KTT, VTT); break;                                                                         \
+    case 8: AMX_EBE_B(8, KTT, VTT); break;                                                                         \
+    case 10: AMX_EBE_B(10, KTT, VTT); break;                                                                       \
+    case 12: AMX_EBE_B(12, KTT, VTT); break;                                                                       \
+    case 20: AMX_EBE_B(20, KTT, VTT); break;                                                                       \
+    case 24: AMX_EBE_B(24, KTT, VTT); break;                                                                       \
+    case 30: AMX_EBE_B(30, KTT, VTT); break;                                                                       \
+    case 60: AMX_EBE_B(60, KTT, VTT); break;                                                                       \
+    default: fail(AMX_NOT_IMPLEMENTED, "batched launches need a tile kernel (n = " + std::to_string(n) + ")");   \
+    }
+    if (vt_double) {
+        AMX_REQUIRE(kt == KType::F64, AMX_INVALID_ARGUMENT, "FP64 vectors need FP64 element matrices");
+        AMX_EBE_N(double, double);
+    } else if (kt == KType::F32) {
+        AMX_EBE_N(float, float);
+    } else {
+        AMX_EBE_N(uint16_t, float);
+    }
+#undef AMX_EBE_N
+#undef AMX_EBE_B
+    AMX_HIP(hipGetLastError());
+}
+
+void launch_gather_batched(bool vt_double, const GatherJob* jobs, const uint32_t* prefix, uint32_t n_jobs,
+                           uint32_t total_blocks, hipStream_t s)
+{
+    if (total_blocks == 0 || n_jobs == 0) return;
+    if (vt_double) gather_batched_kernel<double><<<total_blocks, 256, 0, s>>>(jobs, prefix, n_jobs);
+    else gather_batched_kernel<float><<<total_blocks, 256, 0, s>>>(jobs, prefix, n_jobs);
+    AMX_HIP(hipGetLastError());
+}
+
+bool batched_n(uint32_t n) { return has_tile_kernel(n); }
 
 void launch_bc_fixup(bool vt_double, int bc, const uint32_t* cdofs, uint32_t nc, const double* coef, const void* u,
                      void* v, hipStream_t s)

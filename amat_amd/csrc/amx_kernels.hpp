@@ -61,6 +61,37 @@ void launch_gather(bool vt_double, int bc, const uint32_t* off, const uint32_t* 
                    const void* u, const uint8_t* flag, const double* coef, void* v, uint32_t n, hipStream_t s,
                    const uint32_t* skip = nullptr);
 
+// ---- batched (many operators per launch) -------------------------------
+// One entry per (job, element group); work items are tiles (EbE) or blocks of
+// 256 DoFs (gather), prefix[i] = first item of entry i, prefix[n_jobs] = total.
+struct EbeJob {
+    const void* K;
+    const uint32_t* map;
+    const void* u;
+    void* out;              // the job's ve array
+    uint64_t ve_base;
+    uint32_t tile_begin, n_tiles;
+    const uint32_t* skip;   // the job's converged flag
+    const float* kdiag;
+    uint32_t masked, pad;
+};
+struct GatherJob {
+    const uint32_t* off;
+    const uint32_t* pos;
+    const void* ve;
+    const void* u;
+    const uint8_t* flag;
+    const double* coef;
+    void* v;
+    uint32_t n, bc;
+    const uint32_t* skip;
+};
+bool batched_n(uint32_t n);
+void launch_ebe_batched(uint32_t n, KType kt, bool vt_double, const EbeJob* jobs, const uint32_t* prefix,
+                        uint32_t n_jobs, uint32_t total_tiles, hipStream_t s);
+void launch_gather_batched(bool vt_double, const GatherJob* jobs, const uint32_t* prefix, uint32_t n_jobs,
+                           uint32_t total_blocks, hipStream_t s);
+
 // constrained rows after an atomic scatter or a halo accumulation
 void launch_bc_fixup(bool vt_double, int bc, const uint32_t* cdofs, uint32_t nc, const double* coef,
                      const void* u, void* v, hipStream_t s);

@@ -276,8 +276,9 @@ void stage_block(Op& op, uint32_t eid, const double* m, int layout, uint32_t bi,
 }  // namespace
 
 void ebe_launch(Op& op, KType kt, bool vt_double, bool masked, int phase, const void* u, void* v, hipStream_t s,
-                const uint32_t* skip)
+                const uint32_t* skip, void* ve)
 {
+    if (!ve) ve = op.ve.p;
     const bool atomic = op.scatter == AMX_SCATTER_ATOMIC;
     for (auto& g : op.groups) {
         const int k = (int)kt;
@@ -286,7 +287,7 @@ void ebe_launch(Op& op, KType kt, bool vt_double, bool masked, int phase, const 
         a.K = g.K[k].p;
         a.map = g.map.as<uint32_t>();
         a.u = u;
-        a.out = atomic ? v : op.ve.p;
+        a.out = atomic ? v : ve;
         a.ve_base = g.ve_base;
         a.skip = skip;
         a.kdiag = kt == KType::BF16 ? g.kdiag16.as<float>() : nullptr;
@@ -297,22 +298,24 @@ void ebe_launch(Op& op, KType kt, bool vt_double, bool masked, int phase, const 
     }
 }
 
-void gather_launch(Op& op, bool vt_double, int bc, const void* u, void* v, hipStream_t s, const uint32_t* skip)
+void gather_launch(Op& op, bool vt_double, int bc, const void* u, void* v, hipStream_t s, const uint32_t* skip,
+                   const void* ve)
 {
-    launch_gather(vt_double, bc, op.off.as<uint32_t>(), op.pos.as<uint32_t>(), op.ve.p, u, op.flag.as<uint8_t>(),
-                  op.coef64.as<double>(), v, op.n_total, s, skip);
+    launch_gather(vt_double, bc, op.off.as<uint32_t>(), op.pos.as<uint32_t>(), ve ? ve : op.ve.p, u,
+                  op.flag.as<uint8_t>(), op.coef64.as<double>(), v, op.n_total, s, skip);
 }
 
-void apply(Op& op, KType kt, bool vt_double, int bc, const void* u, void* v, hipStream_t s, const uint32_t* skip)
+void apply(Op& op, KType kt, bool vt_double, int bc, const void* u, void* v, hipStream_t s, const uint32_t* skip,
+           void* ve)
 {
     const bool masked = bc == AMX_BCOP_IDENTITY;
     if (op.scatter == AMX_SCATTER_ATOMIC) {
         launch_fill(v, op.n_total, vt_double, 0.0, s);
-        ebe_launch(op, kt, vt_double, masked, 2, u, v, s, skip);
+        ebe_launch(op, kt, vt_double, masked, 2, u, v, s, skip, ve);
         launch_bc_fixup(vt_double, bc, op.cdofs.as<uint32_t>(), op.n_cdofs, op.coef64.as<double>(), u, v, s);
     } else {
-        ebe_launch(op, kt, vt_double, masked, 2, u, v, s, skip);
-        gather_launch(op, vt_double, bc, u, v, s, skip);
+        ebe_launch(op, kt, vt_double, masked, 2, u, v, s, skip, ve);
+        gather_launch(op, vt_double, bc, u, v, s, skip, ve);
     }
 }
 

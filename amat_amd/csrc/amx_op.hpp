@@ -114,13 +114,19 @@ struct DeviceGuard {
 // tiles, 2 all tiles.  vt_double: FP64 vectors and storage, else FP32 vectors
 // with storage kt (F32 or BF16).
 // skip: optional device flag; a nonzero value turns the launches into no-ops.
+// ve: element-row buffer of the deterministic scatter (NULL: the handle's own;
+// callers running several applications of one operator at once bring theirs,
+// of ve_rows doubles).
 void ebe_launch(Op& op, KType kt, bool vt_double, bool masked, int phase, const void* u, void* v, hipStream_t s,
-                const uint32_t* skip = nullptr);
+                const uint32_t* skip = nullptr, void* ve = nullptr);
 void gather_launch(Op& op, bool vt_double, int bc, const void* u, void* v, hipStream_t s,
-                   const uint32_t* skip = nullptr);
+                   const uint32_t* skip = nullptr, const void* ve = nullptr);
 // complete one-rank operator application: v = A u with BC form bc
 void apply(Op& op, KType kt, bool vt_double, int bc, const void* u, void* v, hipStream_t s,
-           const uint32_t* skip = nullptr);
+           const uint32_t* skip = nullptr, void* ve = nullptr);
+// Jacobi inverse diagonal of the operator solved (constrained rows: 1 for
+// identity rows, the penalty for penalty rows); scratch: n_total doubles.
+void build_dinv(Op& op, double* dinv, double* scratch, hipStream_t s);
 
 }  // namespace amx
 

@@ -18,6 +18,20 @@ using amx::Op;
 
 namespace amx {
 
+// Jacobi diagonal of the operator being solved -> 1/diag.  Constrained rows
+// take the value that BC form puts there: 1 for identity rows, the penalty for
+// penalty rows (apply_bc_diagonal, include/aMatFree.hpp:4309-4337, does the
+// same for BC_IMATRIX / BC_PENALTY).
+void build_dinv(Op& op, double* dinv, double* d, hipStream_t s)
+{
+    AMX_HIP(hipMemcpyAsync(d, op.diag.p, op.n_total * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (op.bc_operator == AMX_BCOP_IDENTITY)
+        launch_set_entries(d, op.cdofs.as<uint32_t>(), op.n_cdofs, 1.0, s);
+    else if (op.bc_operator == AMX_BCOP_PENALTY_DIAG)
+        launch_set_entries(d, op.cdofs.as<uint32_t>(), op.n_cdofs, op.penalty(), s);
+    invert(d, dinv, op.n_total, s);
+}
+
 namespace {
 
 void ensure(DevBuf& b, size_t bytes)
@@ -57,20 +71,7 @@ CgScalars read_scalars(Op& op, const CgScalars* dsc, hipStream_t s)
     return *op.host_sc;
 }
 
-// Jacobi diagonal of the operator being solved -> 1/diag.  Constrained rows
-// take the value that BC form puts there: 1 for identity rows, the penalty for
-// penalty rows (apply_bc_diagonal, include/aMatFree.hpp:4309-4337, does the
-// same for BC_IMATRIX / BC_PENALTY).
-void build_dinv(Op& op, hipStream_t s)
-{
-    double* d = op.ws_q.as<double>();
-    AMX_HIP(hipMemcpyAsync(d, op.diag.p, op.n_total * sizeof(double), hipMemcpyDeviceToDevice, s));
-    if (op.bc_operator == AMX_BCOP_IDENTITY)
-        launch_set_entries(d, op.cdofs.as<uint32_t>(), op.n_cdofs, 1.0, s);
-    else if (op.bc_operator == AMX_BCOP_PENALTY_DIAG)
-        launch_set_entries(d, op.cdofs.as<uint32_t>(), op.n_cdofs, op.penalty(), s);
-    invert(d, op.ws_dinv.as<double>(), op.n_total, s);
-}
+void build_dinv_ws(Op& op, hipStream_t s) { amx::build_dinv(op, op.ws_dinv.as<double>(), op.ws_q.as<double>(), s); }
 
 // PCG on A x = b (x starts at 0), vectors of type f64 ? double : float, the
 // operator with storage kt and the handle's BC form.  Returns iterations.
@@ -118,7 +119,7 @@ int amx_cg_solve_dev(amx_op* h, const double* b, double* x, double rtol, int max
         hipStream_t s = op.pick(stream);
         const auto t0 = std::chrono::steady_clock::now();
         amx::ensure_ws(op);
-        amx::build_dinv(op, s);
+        amx::build_dinv_ws(op, s);
         amx::CgScalars* dsc = op.ws_sc.as<amx::CgScalars>();
         const amx::CgScalars sc = amx::pcg(op, KType::F64, true, b, x, op.ws_r.p, op.ws_z.p, op.ws_p.p, op.ws_q.p,
                                            rtol, max_it, dsc, s);
@@ -156,13 +157,14 @@ int amx_ir_cg_solve_dev(amx_op* h, int precision, const double* b, double* x, do
         hipStream_t s = op.pick(stream);
         const auto t0 = std::chrono::steady_clock::now();
         amx::ensure_ws(op);
-        amx::build_dinv(op, s);
+        amx::build_dinv_ws(op, s);
         const KType kt = precision == AMX_PREC_FP32 ? KType::F32 : KType::BF16;
         amx::CgScalars* outer = op.ws_sc.as<amx::CgScalars>();
         amx::CgScalars* inner = outer + 1;
         const uint32_t n = op.n_total;
         double* partials = op.ws_partials.as<double>();
-        // x = 0, r = b
+        // x = 0, r = b (the outer scalars start zeroed: their reduction counter included)
+        amx::cg_reset(outer, rtol, (uint32_t)max_outer, s);
         amx::launch_fill(x, n, true, 0.0, s);
         amx::launch_fill(op.ws_q.p, n, true, 0.0, s);
         amx::residual(b, op.ws_q.as<double>(), op.ws_r.as<double>(), n, partials, outer, s);

@@ -32,29 +32,37 @@ __device__ __forceinline__ double block_sum(double x, double* sh)
 
 // Deposit this block's NV partial sums; return true in the block that must
 // finish the reduction (all partials visible to it), with the totals in tot.
+// blk / nblk: this block's index among the reduction's blocks and their count.
 template <int NV>
-__device__ bool reduce_partials(const double (&v)[NV], double* partials, uint32_t* counter, double (&tot)[NV], double* sh)
+__device__ bool reduce_partials_n(const double (&v)[NV], double* partials, uint32_t* counter, double (&tot)[NV],
+                                  double* sh, uint32_t blk, uint32_t nblk)
 {
     __shared__ bool last;
     double b[NV];
     for (int k = 0; k < NV; k++) b[k] = block_sum(v[k], sh);
     if (threadIdx.x == 0) {
-        for (int k = 0; k < NV; k++) partials[(size_t)k * gridDim.x + blockIdx.x] = b[k];
+        for (int k = 0; k < NV; k++) partials[(size_t)k * nblk + blk] = b[k];
         __threadfence();
         const uint32_t prev = atomicAdd(counter, 1u);
-        last = (prev == gridDim.x - 1);
+        last = (prev == nblk - 1);
     }
     __syncthreads();
     if (!last) return false;
     __threadfence();
     for (int k = 0; k < NV; k++) {
         double s = 0.0;
-        for (uint32_t i = threadIdx.x; i < gridDim.x; i += kBlock)
-            s += __hip_atomic_load(&partials[(size_t)k * gridDim.x + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (uint32_t i = threadIdx.x; i < nblk; i += kBlock)
+            s += __hip_atomic_load(&partials[(size_t)k * nblk + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         tot[k] = block_sum(s, sh);
     }
     if (threadIdx.x == 0) *counter = 0u;
     return true;
+}
+
+template <int NV>
+__device__ bool reduce_partials(const double (&v)[NV], double* partials, uint32_t* counter, double (&tot)[NV], double* sh)
+{
+    return reduce_partials_n<NV>(v, partials, counter, tot, sh, blockIdx.x, gridDim.x);
 }
 
 template <typename VT>
@@ -194,6 +202,119 @@ __global__ void invert_kernel(const double* __restrict__ d, double* __restrict__
     if (i < n) dinv[i] = 1.0 / d[i];
 }
 
+// ---- batched: many jobs per launch ---------------------------------------
+// Block -> (job j, local block lb) through the block prefix; every job keeps
+// the grid of its single-job launch (reduction_grid(n) blocks, same stride),
+// so its sums are bit-identical to a solve of that job alone.
+__device__ __forceinline__ uint32_t job_of(const uint32_t* __restrict__ prefix, uint32_t n, uint32_t x)
+{
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void cg_pq_batched_kernel(const CgJob* __restrict__ jobs,
+                                                              const uint32_t* __restrict__ prefix, uint32_t n_jobs)
+{
+    const uint32_t gb = __builtin_amdgcn_readfirstlane(blockIdx.x);
+    const uint32_t j = job_of(prefix, n_jobs, gb);
+    const CgJob& jb = jobs[j];
+    CgScalars* sc = jb.sc;
+    if (sc->done) return;
+    __shared__ double sh[kBlock];
+    const uint32_t lb = gb - prefix[j], nblk = jb.nblk, n = jb.n;
+    const VT* p = static_cast<const VT*>(jb.p);
+    const VT* q = static_cast<const VT*>(jb.q);
+    double acc[1] = {0.0};
+    for (uint32_t i = lb * kBlock + threadIdx.x; i < n; i += nblk * kBlock) acc[0] += (double)p[i] * (double)q[i];
+    double tot[1];
+    if (!reduce_partials_n<1>(acc, jb.partials, &sc->counter, tot, sh, lb, nblk)) return;
+    if (threadIdx.x == 0) {
+        sc->pq = tot[0];
+        if (!(tot[0] > 0.0)) {
+            sc->breakdown = 1u;
+            sc->done = 1u;
+        } else {
+            sc->alpha = sc->rz / tot[0];
+        }
+    }
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void cg_update_batched_kernel(const CgJob* __restrict__ jobs,
+                                                                  const uint32_t* __restrict__ prefix, uint32_t n_jobs)
+{
+    const uint32_t gb = __builtin_amdgcn_readfirstlane(blockIdx.x);
+    const uint32_t j = job_of(prefix, n_jobs, gb);
+    const CgJob& jb = jobs[j];
+    CgScalars* sc = jb.sc;
+    if (sc->done) return;
+    __shared__ double sh[kBlock];
+    const uint32_t lb = gb - prefix[j], nblk = jb.nblk, n = jb.n;
+    VT* x = static_cast<VT*>(jb.x);
+    VT* r = static_cast<VT*>(jb.r);
+    VT* z = static_cast<VT*>(jb.z);
+    const VT* p = static_cast<const VT*>(jb.p);
+    const VT* q = static_cast<const VT*>(jb.q);
+    const double* dinv = jb.dinv;
+    const double alpha = sc->alpha;
+    double acc[2] = {0.0, 0.0};
+    for (uint32_t i = lb * kBlock + threadIdx.x; i < n; i += nblk * kBlock) {
+        x[i] = (VT)((double)x[i] + alpha * (double)p[i]);
+        const double ri = (double)r[i] - alpha * (double)q[i];
+        const VT rv = (VT)ri;
+        const VT zv = (VT)(dinv[i] * (double)rv);
+        r[i] = rv;
+        z[i] = zv;
+        acc[0] += (double)rv * (double)zv;
+        acc[1] += (double)rv * (double)rv;
+    }
+    double tot[2];
+    if (!reduce_partials_n<2>(acc, jb.partials, &sc->counter, tot, sh, lb, nblk)) return;
+    if (threadIdx.x == 0) {
+        sc->iter += 1;
+        sc->rr = tot[1];
+        if (tot[1] <= sc->tol2 * sc->bb || sc->iter >= sc->max_it) {
+            sc->done = 1u;
+        } else {
+            sc->beta = tot[0] / sc->rz;
+            sc->rz = tot[0];
+        }
+    }
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void cg_p_batched_kernel(const CgJob* __restrict__ jobs,
+                                                             const uint32_t* __restrict__ prefix, uint32_t n_jobs)
+{
+    const uint32_t gb = __builtin_amdgcn_readfirstlane(blockIdx.x);
+    const uint32_t j = job_of(prefix, n_jobs, gb);
+    const CgJob& jb = jobs[j];
+    const CgScalars* sc = jb.sc;
+    if (sc->done) return;
+    const uint32_t lb = gb - prefix[j], nblk = jb.nblk, n = jb.n;
+    VT* p = static_cast<VT*>(jb.p);
+    const VT* z = static_cast<const VT*>(jb.z);
+    const double beta = sc->beta;
+    for (uint32_t i = lb * kBlock + threadIdx.x; i < n; i += nblk * kBlock) p[i] = (VT)((double)z[i] + beta * (double)p[i]);
+}
+
+// restart a CG: scalars zeroed, tolerance and iteration limit set (a kernel,
+// so it can sit in a stream or a graph without host staging)
+__global__ void cg_reset_kernel(CgScalars* sc, double tol2, uint32_t max_it)
+{
+    if (threadIdx.x != 0) return;
+    CgScalars h{};
+    h.tol2 = tol2;
+    h.max_it = max_it;
+    *sc = h;
+}
+
 }  // namespace
 
 uint32_t reduction_grid(uint32_t n)
@@ -251,6 +372,39 @@ void scale_to_f32(const double* r, float* r32, uint32_t n, const CgScalars* sc, 
 void correct(double* x, const float* d32, uint32_t n, const CgScalars* sc, hipStream_t s)
 {
     correct_kernel<<<reduction_grid(n), kBlock, 0, s>>>(x, d32, n, sc);
+    AMX_HIP(hipGetLastError());
+}
+
+void cg_pq_batched(bool f64, const CgJob* jobs, const uint32_t* prefix, uint32_t n_jobs, uint32_t total_blocks,
+                   hipStream_t s)
+{
+    if (!total_blocks) return;
+    if (f64) cg_pq_batched_kernel<double><<<total_blocks, kBlock, 0, s>>>(jobs, prefix, n_jobs);
+    else cg_pq_batched_kernel<float><<<total_blocks, kBlock, 0, s>>>(jobs, prefix, n_jobs);
+    AMX_HIP(hipGetLastError());
+}
+
+void cg_update_batched(bool f64, const CgJob* jobs, const uint32_t* prefix, uint32_t n_jobs, uint32_t total_blocks,
+                       hipStream_t s)
+{
+    if (!total_blocks) return;
+    if (f64) cg_update_batched_kernel<double><<<total_blocks, kBlock, 0, s>>>(jobs, prefix, n_jobs);
+    else cg_update_batched_kernel<float><<<total_blocks, kBlock, 0, s>>>(jobs, prefix, n_jobs);
+    AMX_HIP(hipGetLastError());
+}
+
+void cg_p_batched(bool f64, const CgJob* jobs, const uint32_t* prefix, uint32_t n_jobs, uint32_t total_blocks,
+                  hipStream_t s)
+{
+    if (!total_blocks) return;
+    if (f64) cg_p_batched_kernel<double><<<total_blocks, kBlock, 0, s>>>(jobs, prefix, n_jobs);
+    else cg_p_batched_kernel<float><<<total_blocks, kBlock, 0, s>>>(jobs, prefix, n_jobs);
+    AMX_HIP(hipGetLastError());
+}
+
+void cg_reset(CgScalars* sc, double rtol, uint32_t max_it, hipStream_t s)
+{
+    cg_reset_kernel<<<1, 64, 0, s>>>(sc, rtol * rtol, max_it);
     AMX_HIP(hipGetLastError());
 }
 

@@ -1,0 +1,329 @@
+"""Serving CG solves on one GPU: request stream, HBM-budget admission and the
+continuous-batching loop around the native server (amx_server_* in
+include/amat_amd.h).
+
+The engine (native): every active job advances by `n_iter` CG iterations per
+step, each phase of the iteration one launch over all jobs; a job's result is
+bit-identical alone and in any mix.  The policy (here): requests arrive on a
+clock; a request is admitted when a job slot is free and its operator plus its
+Krylov workspace fit the HBM budget next to everything already resident.
+Operators (element matrices, maps, BC tables) stay resident in an LRU cache
+keyed by (kind, n) and are evicted, least recently used first, only when no
+running job uses them and room is needed.
+
+Request mix of the serving workload: 60% hex8 Poisson n^3 with n ~ U{16..96},
+30% hex20 elasticity n ~ U{8..40} with E_e ~ U[0.5, 2), 10% jittered tet10
+Poisson n ~ U{12..48}; each request has its own seeded RHS in [-1, 1)
+(constrained rows 0) and is solved to 1e-10 relative residual.
+"""
+from __future__ import annotations
+
+import collections
+import ctypes as C
+import dataclasses
+import math
+import time
+from typing import Callable
+
+import numpy as np
+
+from . import _capi
+from ._capi import call
+from .amat import BCOP, PREC, Maps, _ptr, aMatFree
+from . import mesh as meshmod, problems
+
+MIX = (("hex8", 0.6, 16, 96), ("hex20", 0.3, 8, 40), ("tet10", 0.1, 12, 48))
+
+
+class Server:
+    """The native continuous-batching CG server (amx_server_*)."""
+
+    def __init__(self, device: int = 0, max_jobs: int = 64):
+        h = C.c_void_p()
+        call("amx_server_create", int(device), int(max_jobs), C.byref(h))
+        self._h = h
+        self.max_jobs = max_jobs
+        self._ops = {}  # job id -> operator (kept alive while the job runs)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _capi.lib().amx_server_destroy(h)
+            except Exception:  # interpreter shutdown
+                pass
+            self._h = None
+
+    @staticmethod
+    def job_bytes(op: aMatFree, method: PREC) -> int:
+        v = C.c_uint64()
+        call("amx_server_job_bytes", op.handle, int(method), C.byref(v))
+        return int(v.value)
+
+    def submit(self, op: aMatFree, method: PREC, b: np.ndarray, rtol: float = 1e-10, inner_rtol: float = 1e-2,
+               max_it: int = 20000, max_outer: int = 60) -> int:
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        jid = C.c_int64()
+        call("amx_server_submit", self._h, op.handle, int(method), _ptr(b), float(rtol), float(inner_rtol),
+             int(max_it), int(max_outer), C.byref(jid))
+        self._ops[jid.value] = op
+        return int(jid.value)
+
+    def step(self, n_iter: int = 8) -> int:
+        k = C.c_int32()
+        call("amx_server_step", self._h, int(n_iter), C.byref(k))
+        return k.value
+
+    def poll(self) -> list[int]:
+        buf = (C.c_int64 * self.max_jobs)()
+        k = C.c_int32()
+        out = []
+        while True:
+            call("amx_server_poll", self._h, buf, self.max_jobs, C.byref(k))
+            out.extend(buf[i] for i in range(k.value))
+            if k.value < self.max_jobs:
+                return out
+
+    def result(self, job_id: int, n: int | None = None):
+        """(x or None, SolveStats); releases the job."""
+        st = _capi.SolveStats()
+        x = None
+        if n is not None:
+            x = np.empty(n, dtype=np.float64)
+        call("amx_server_result", self._h, C.c_int64(job_id), _ptr(x), C.byref(st))
+        self._ops.pop(job_id, None)
+        return x, st
+
+    def info(self) -> _capi.ServerInfo:
+        i = _capi.ServerInfo()
+        call("amx_server_get_info", self._h, C.byref(i))
+        return i
+
+
+# ----------------------------------------------------------------- workload
+@dataclasses.dataclass
+class Request:
+    rid: int
+    kind: str
+    n: int
+    arrival: float  # seconds after the start of the stream
+    seed: int
+
+
+def make_requests(count: int, rate: float, seed: int = 7, mix=MIX) -> list[Request]:
+    """Poisson arrivals at `rate` per second (exponential gaps) with the kind
+    and size mix drawn from counter-based uniforms (reproducible)."""
+    u_gap = meshmod.uniform(seed, 11, count, 0.0, 1.0)
+    u_kind = meshmod.uniform(seed, 12, count, 0.0, 1.0)
+    u_size = meshmod.uniform(seed, 13, count, 0.0, 1.0)
+    t = 0.0
+    out = []
+    cum = np.cumsum([m[1] for m in mix])
+    for i in range(count):
+        t += -math.log(1.0 - u_gap[i]) / rate
+        k = int(np.searchsorted(cum, u_kind[i] * cum[-1], side="right"))
+        kind, _, lo, hi = mix[min(k, len(mix) - 1)]
+        n = lo + min(int(u_size[i] * (hi - lo + 1)), hi - lo)
+        out.append(Request(i, kind, n, t, seed * 100003 + i))
+    return out
+
+
+def build_problem(kind: str, n: int) -> problems.Problem:
+    if kind == "hex8":
+        return problems.poisson_hex8(n)
+    if kind == "hex20":
+        return problems.elasticity_hex20(n, seed=1000 + n)
+    if kind == "tet10":
+        return problems.poisson_tet10(n, seed=2000 + n, with_rhs=False)
+    raise ValueError(kind)
+
+
+def rhs_for(p: problems.Problem, seed: int) -> np.ndarray:
+    b = meshmod.uniform(seed, 5, p.n_dofs, -1.0, 1.0)
+    b[p.constrained.astype(np.int64)] = 0.0
+    return b
+
+
+def build_operator(p: problems.Problem, precision=PREC.FP64 | PREC.FP32 | PREC.BF16) -> aMatFree:
+    op = p.build_operator(aMatFree, Maps, precision=precision, bcop=BCOP.IDENTITY)
+    return op
+
+
+# ------------------------------------------------------------------ policy
+@dataclasses.dataclass
+class CachedOp:
+    key: tuple
+    problem: problems.Problem
+    op: aMatFree
+    bytes: int
+    users: int = 0
+
+
+class OperatorCache:
+    """Resident operators, LRU order; evicts unused ones to make room."""
+
+    def __init__(self, builder: Callable[[str, int], tuple] | None = None):
+        self.items: collections.OrderedDict[tuple, CachedOp] = collections.OrderedDict()
+        self.builder = builder or self._default_builder
+        self.builds = 0
+        self.build_seconds = 0.0
+        self.evictions = 0
+
+    @staticmethod
+    def _default_builder(kind: str, n: int):
+        p = build_problem(kind, n)
+        op = build_operator(p)
+        return p, op, int(op.info().device_bytes)
+
+    @property
+    def bytes(self) -> int:
+        return sum(c.bytes for c in self.items.values())
+
+    def get(self, key):
+        c = self.items.get(key)
+        if c is not None:
+            self.items.move_to_end(key)
+        return c
+
+    def evict_for(self, need: int, budget: int, other: int) -> bool:
+        """Evict unused operators (LRU first) until `need` more bytes fit."""
+        for key in list(self.items.keys()):
+            if self.bytes + other + need <= budget:
+                break
+            c = self.items[key]
+            if c.users == 0:
+                del self.items[key]
+                self.evictions += 1
+        return self.bytes + other + need <= budget
+
+    def build(self, key) -> CachedOp:
+        t0 = time.perf_counter()
+        p, op, nbytes = self.builder(*key)
+        self.build_seconds += time.perf_counter() - t0
+        self.builds += 1
+        c = CachedOp(key, p, op, nbytes)
+        self.items[key] = c
+        return c
+
+
+def estimate_op_bytes(kind: str, n: int) -> int:
+    """Device bytes of an operator of this kind and size with FP64 + FP32 +
+    BF16 storage (element matrices, maps, deterministic-scatter tables), used
+    before the operator exists."""
+    if kind == "hex8":
+        E, nd, N = n ** 3, 8, (n + 1) ** 3
+    elif kind == "hex20":
+        E, nd, N = n ** 3, 60, 3 * ((n + 1) ** 3 + 3 * n * (n + 1) ** 2)
+    else:
+        E, nd = 6 * n ** 3, 10
+        N = (n + 1) ** 3 + 7 * n ** 3 + 3 * n * (n + 1) ** 2
+    return E * nd * nd * 14 + E * nd * (4 + 4 + 4 + 8 + 4) + N * 40
+
+
+def estimate_job_bytes(kind: str, n: int, method: PREC) -> int:
+    if kind == "hex8":
+        E, nd, N = n ** 3, 8, (n + 1) ** 3
+    elif kind == "hex20":
+        E, nd, N = n ** 3, 60, 3 * ((n + 1) ** 3 + 3 * n * (n + 1) ** 2)
+    else:
+        E, nd = 6 * n ** 3, 10
+        N = (n + 1) ** 3 + 7 * n ** 3 + 3 * n * (n + 1) ** 2
+    ve = E * nd * 8
+    return (7 * N * 8 if method == PREC.FP64 else 5 * N * 8 + 6 * N * 4) + ve + 3 * 1024 * 8
+
+
+@dataclasses.dataclass
+class Done:
+    rid: int
+    kind: str
+    n: int
+    arrival: float
+    admitted: float
+    finished: float
+    iterations: int
+    outer: int
+    rel_residual: float
+    converged: bool
+    x: np.ndarray | None = None
+
+
+class ServingLoop:
+    """Admission by HBM budget + continuous batching on one Server.
+
+    Each turn: admit arrived requests (FIFO) while a slot is free and the
+    operator + workspace fit the budget; run one step of the server; retire
+    finished jobs.  Latency = finish - arrival."""
+
+    def __init__(self, server: Server, cache: OperatorCache, method: PREC, budget_bytes: int, n_iter: int = 8,
+                 rtol: float = 1e-10, inner_rtol: float = 1e-2, keep_x: bool = False,
+                 clock: Callable[[], float] = time.perf_counter):
+        self.server = server
+        self.cache = cache
+        self.method = PREC(method)
+        self.budget = int(budget_bytes)
+        self.n_iter = n_iter
+        self.rtol = rtol
+        self.inner_rtol = inner_rtol
+        self.keep_x = keep_x
+        self.clock = clock
+
+    def run(self, requests: list[Request], realtime: bool = True) -> dict:
+        pending = collections.deque()
+        incoming = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        running = {}  # job id -> (request, CachedOp, admitted time, workspace bytes)
+        done: list[Done] = []
+        workspace = 0
+        t0 = self.clock()
+        peak_jobs = 0
+        peak_bytes = 0
+        while incoming or pending or running:
+            now = self.clock() - t0
+            while incoming and (incoming[0].arrival <= now or not realtime):
+                pending.append(incoming.popleft())
+            # admission (FIFO, head-of-line: a request that does not fit waits)
+            while pending and len(running) < self.server.max_jobs:
+                r = pending[0]
+                key = (r.kind, r.n)
+                c = self.cache.get(key)
+                need_op = 0 if c is not None else estimate_op_bytes(r.kind, r.n)
+                need_job = estimate_job_bytes(r.kind, r.n, self.method)
+                if not self.cache.evict_for(need_op + need_job, self.budget, workspace):
+                    if not running:
+                        raise RuntimeError(f"request {r.rid} ({r.kind} n={r.n}) cannot fit the HBM budget")
+                    break
+                if c is None:
+                    c = self.cache.build(key)
+                b = rhs_for(c.problem, r.seed)
+                jid = self.server.submit(c.op, self.method, b, rtol=self.rtol, inner_rtol=self.inner_rtol)
+                c.users += 1
+                jb = self.server.job_bytes(c.op, self.method)
+                workspace += jb
+                running[jid] = (r, c, self.clock() - t0, jb)
+                pending.popleft()
+            peak_jobs = max(peak_jobs, len(running))
+            peak_bytes = max(peak_bytes, workspace + self.cache.bytes)
+            if running:
+                self.server.step(self.n_iter)
+                for jid in self.server.poll():
+                    r, c, t_adm, jb = running.pop(jid)
+                    x, st = self.server.result(jid, c.problem.n_dofs if self.keep_x else None)
+                    c.users -= 1
+                    workspace -= jb
+                    done.append(Done(r.rid, r.kind, r.n, r.arrival, t_adm, self.clock() - t0, st.iterations,
+                                     st.outer_iterations, st.rel_residual, bool(st.converged), x))
+            elif incoming and realtime:
+                time.sleep(max(0.0, min(incoming[0].arrival - (self.clock() - t0), 0.01)))
+        makespan = self.clock() - t0
+        lat = np.array([d.finished - d.arrival for d in done])
+        return {
+            "done": done,
+            "solves": len(done),
+            "makespan_s": makespan,
+            "solves_per_s": len(done) / makespan if makespan > 0 else 0.0,
+            "p50_s": float(np.percentile(lat, 50)) if len(lat) else 0.0,
+            "p99_s": float(np.percentile(lat, 99)) if len(lat) else 0.0,
+            "all_converged": all(d.converged for d in done),
+            "max_rel_residual": max((d.rel_residual for d in done), default=0.0),
+            "peak_jobs": peak_jobs,
+            "peak_resident_bytes": peak_bytes,
+        }

@@ -349,6 +349,47 @@ int amx_mrhs_autotune(amx_mrhs* m, int nrhs, const float* const* U, float* const
 int amx_mrhs_graph_capture(amx_mrhs* m, int nrhs, const float* const* U, float* const* V, void* stream);
 int amx_mrhs_graph_launch(amx_mrhs* m, void* stream);
 
+/* ------------------------------------------------------- serving
+ * A continuous-batching CG service on one device (no counterpart in the
+ * reference, whose solves go one at a time through par::solve,
+ * examples/include/solve.hpp:31-83).  Jobs are CG solves A x = b on finalized
+ * one-rank operators with the deterministic scatter; each keeps its vectors and
+ * scalars resident on the device.  amx_server_step runs n_iter iterations of
+ * every active job, each phase as one launch over all jobs, then advances every
+ * job's state (one device-to-host copy of the scalars).  A job's result is
+ * bit-identical alone, in any mix, and to amx_cg_solve_dev (method
+ * AMX_PREC_FP64) or amx_ir_cg_solve_dev (AMX_PREC_FP32 / AMX_PREC_BF16: FP64
+ * iterative refinement, inner Jacobi-PCG on that storage).  Operators must
+ * outlive their jobs; one host thread per server. */
+typedef struct amx_server amx_server;
+typedef struct amx_server_info {
+    uint32_t active_jobs;   /* admitted, not finished */
+    uint32_t held_jobs;     /* admitted, result not fetched */
+    uint32_t free_slots;
+    uint32_t pad;
+    uint64_t workspace_bytes; /* device bytes of the held jobs (operators not counted) */
+    uint64_t steps, iterations, completed;
+} amx_server_info;
+int amx_server_create(int device, int max_jobs, amx_server** out);
+int amx_server_destroy(amx_server* server);
+/* Device bytes one job of this method on this operator holds (for admission
+ * by memory budget; the operator's own bytes are in amx_op_info). */
+int amx_server_job_bytes(const amx_op* op, int method, uint64_t* bytes);
+/* Admit a job now: copies b (n_total doubles), allocates its workspace and
+ * initializes it; AMX_REJECTED when every slot is taken.  inner_rtol and
+ * max_outer apply to refinement; max_it bounds each CG. */
+int amx_server_submit(amx_server* server, amx_op* op, int method, const double* b, double rtol,
+                      double inner_rtol, int max_it, int max_outer, int64_t* job_id);
+/* n_iter batched iterations of all active jobs, then state advance;
+ * *n_finished = jobs that finished in this step. */
+int amx_server_step(amx_server* server, int n_iter, int* n_finished);
+/* ids of finished jobs not yet reported (at most max_ids). */
+int amx_server_poll(amx_server* server, int64_t* job_ids, int max_ids, int* n_ids);
+/* Copy a finished job's solution (n_total doubles, may be NULL) and stats,
+ * then release the job and its device memory. */
+int amx_server_result(amx_server* server, int64_t job_id, double* x, amx_solve_stats* stats);
+int amx_server_get_info(const amx_server* server, amx_server_info* info);
+
 /* ------------------------------------------------------- element kernels
  * Element matrices of the examples (examples/src/ke_matrix.cpp,
  * examples/src/fe_vector.cpp), FP64 on the host.  Row-major output. */
