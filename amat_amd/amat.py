@@ -64,6 +64,10 @@ class PREC(enum.IntFlag):
     BF16 = 4
 
 
+# refinement option: inner CG vectors in FP64 (or-ed into ir_cg's precision)
+IR_F64_VECTORS = 0x100
+
+
 class SCATTER(enum.IntEnum):
     DETERMINISTIC = 0
     ATOMIC = 1

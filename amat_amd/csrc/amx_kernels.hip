@@ -599,8 +599,10 @@ void launch_ebe(const Layout& L, KType kt, bool vt_double, bool masked, bool ato
 {
     if (a.n_tiles == 0) return;
     if (vt_double) {
-        AMX_REQUIRE(kt == KType::F64, AMX_INVALID_ARGUMENT, "FP64 vectors need FP64 element matrices");
-        dispatch_n<double, double>(L, masked, atomic, a, s);
+        // FP64 vectors: FP64 storage, or FP32 / BF16 storage widened on load
+        if (kt == KType::F64) dispatch_n<double, double>(L, masked, atomic, a, s);
+        else if (kt == KType::F32) dispatch_n<float, double>(L, masked, atomic, a, s);
+        else dispatch_n<uint16_t, double>(L, masked, atomic, a, s);
     } else if (kt == KType::F32) {
         dispatch_n<float, float>(L, masked, atomic, a, s);
     } else if (kt == KType::BF16) {
@@ -650,8 +652,13 @@ void launch_ebe_batched(uint32_t n, KType kt, bool vt_double, const EbeJob* jobs
     default: fail(AMX_NOT_IMPLEMENTED, "batched launches need a tile kernel (n = " + std::to_string(n) + ")");   \
     }
     if (vt_double) {
-        AMX_REQUIRE(kt == KType::F64, AMX_INVALID_ARGUMENT, "FP64 vectors need FP64 element matrices");
-        AMX_EBE_N(double, double);
+        if (kt == KType::F64) {
+            AMX_EBE_N(double, double);
+        } else if (kt == KType::F32) {
+            AMX_EBE_N(float, double);
+        } else {
+            AMX_EBE_N(uint16_t, double);
+        }
     } else if (kt == KType::F32) {
         AMX_EBE_N(float, float);
     } else {

@@ -78,6 +78,7 @@ struct Op {
     DevBuf scratch_u, scratch_v;         // host-array API staging
     // solver workspace (allocated on first solve)
     DevBuf ws_r, ws_z, ws_p, ws_q, ws_dinv, ws_partials, ws_sc, ws_x32, ws_b32, ws_r32, ws_z32, ws_p32, ws_q32;
+    DevBuf ws_ib, ws_ix, ws_ir, ws_iz, ws_ip, ws_iq;  // refinement with FP64 inner vectors
     CgScalars* host_sc = nullptr;        // pinned copy of the device scalars
     uint64_t device_bytes = 0;
 

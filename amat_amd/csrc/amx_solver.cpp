@@ -145,8 +145,10 @@ int amx_ir_cg_solve_dev(amx_op* h, int precision, const double* b, double* x, do
 {
     return guarded(__func__, [&] {
         AMX_REQUIRE(h && b && x, AMX_INVALID_ARGUMENT, "NULL argument");
+        const bool v64 = (precision & AMX_IR_F64_VECTORS) != 0;
+        precision &= ~AMX_IR_F64_VECTORS;
         AMX_REQUIRE(precision == AMX_PREC_FP32 || precision == AMX_PREC_BF16, AMX_INVALID_ARGUMENT,
-                    "inner precision must be FP32 or BF16");
+                    "inner precision must be FP32 or BF16 (optionally | AMX_IR_F64_VECTORS)");
         AMX_REQUIRE(rtol > 0.0 && inner_rtol > 0.0 && inner_rtol < 1.0 && max_outer > 0 && max_inner > 0,
                     AMX_INVALID_ARGUMENT, "bad tolerances or iteration limits");
         Op& op = h->op;
@@ -157,6 +159,9 @@ int amx_ir_cg_solve_dev(amx_op* h, int precision, const double* b, double* x, do
         hipStream_t s = op.pick(stream);
         const auto t0 = std::chrono::steady_clock::now();
         amx::ensure_ws(op);
+        if (v64)
+            for (DevBuf* v : {&op.ws_ib, &op.ws_ix, &op.ws_ir, &op.ws_iz, &op.ws_ip, &op.ws_iq})
+                amx::ensure(*v, (size_t)std::max<uint32_t>(op.n_total, 1) * sizeof(double));
         amx::build_dinv_ws(op, s);
         const KType kt = precision == AMX_PREC_FP32 ? KType::F32 : KType::BF16;
         amx::CgScalars* outer = op.ws_sc.as<amx::CgScalars>();
@@ -175,14 +180,22 @@ int amx_ir_cg_solve_dev(amx_op* h, int precision, const double* b, double* x, do
             const amx::CgScalars o = amx::read_scalars(op, outer, s);
             rel = o.bb > 0 ? std::sqrt(o.rr / o.bb) : 0.0;
             if (rel <= rtol || outer_it >= max_outer) break;
-            // inner solve A_low d = r / ||r|| (FP32 vectors, FP64 scalars)
-            amx::scale_to_f32(op.ws_r.as<double>(), op.ws_b32.as<float>(), n, outer, s);
-            const amx::CgScalars in = amx::pcg(op, kt, false, op.ws_b32.p, op.ws_x32.p, op.ws_r32.p, op.ws_z32.p,
-                                               op.ws_p32.p, op.ws_q32.p, inner_rtol, max_inner, inner, s);
+            // inner solve A_low d = r / ||r|| (FP32 or FP64 vectors, FP64 scalars)
+            amx::CgScalars in;
+            if (v64) {
+                amx::scale_to_f64(op.ws_r.as<double>(), op.ws_ib.as<double>(), n, outer, s);
+                in = amx::pcg(op, kt, true, op.ws_ib.p, op.ws_ix.p, op.ws_ir.p, op.ws_iz.p, op.ws_ip.p, op.ws_iq.p,
+                              inner_rtol, max_inner, inner, s);
+            } else {
+                amx::scale_to_f32(op.ws_r.as<double>(), op.ws_b32.as<float>(), n, outer, s);
+                in = amx::pcg(op, kt, false, op.ws_b32.p, op.ws_x32.p, op.ws_r32.p, op.ws_z32.p, op.ws_p32.p,
+                              op.ws_q32.p, inner_rtol, max_inner, inner, s);
+            }
             total_inner += (int)in.iter;
             breakdown = breakdown || in.breakdown;
             // x += ||r|| d ; r = b - A x with the FP64 operator
-            amx::correct(x, op.ws_x32.as<float>(), n, outer, s);
+            if (v64) amx::correct64(x, op.ws_ix.as<double>(), n, outer, s);
+            else amx::correct(x, op.ws_x32.as<float>(), n, outer, s);
             amx::apply(op, KType::F64, true, op.bc_operator, x, op.ws_q.p, s);
             amx::residual(b, op.ws_q.as<double>(), op.ws_r.as<double>(), n, partials, outer, s);
             outer_it++;
@@ -192,7 +205,7 @@ int amx_ir_cg_solve_dev(amx_op* h, int precision, const double* b, double* x, do
             stats->iterations = total_inner;
             stats->outer_iterations = outer_it;
             stats->converged = rel <= rtol;
-            stats->precision = precision;
+            stats->precision = precision | (v64 ? AMX_IR_F64_VECTORS : 0);
             stats->rel_residual = rel;
             stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         }

@@ -36,6 +36,8 @@ void cg_p(bool f64, void* p, const void* z, uint32_t n, const CgScalars* sc, hip
 void residual(const double* b, const double* ax, double* r, uint32_t n, double* partials, CgScalars* sc, hipStream_t s);
 void scale_to_f32(const double* r, float* r32, uint32_t n, const CgScalars* sc, hipStream_t s);
 void correct(double* x, const float* d32, uint32_t n, const CgScalars* sc, hipStream_t s);
+void scale_to_f64(const double* r, double* rs, uint32_t n, const CgScalars* sc, hipStream_t s);
+void correct64(double* x, const double* d, uint32_t n, const CgScalars* sc, hipStream_t s);
 void invert(const double* d, double* dinv, uint32_t n, hipStream_t s);
 // batched iteration pieces: prefix[j] = first block of job j (prefix[n_jobs] = total_blocks)
 void cg_pq_batched(bool f64, const CgJob* jobs, const uint32_t* prefix, uint32_t n_jobs, uint32_t total_blocks,

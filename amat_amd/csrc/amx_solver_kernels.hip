@@ -182,18 +182,20 @@ __global__ __launch_bounds__(kBlock) void residual_kernel(const double* __restri
     }
 }
 
-// r32 = r / s (s = ||r||), diagonal inverse in FP64 kept
-__global__ void scale_to_f32_kernel(const double* __restrict__ r, float* __restrict__ r32, uint32_t n, const CgScalars* sc)
+// rs = r / s (s = ||r||) in the inner solve's vector type
+template <typename VT>
+__global__ void scale_kernel(const double* __restrict__ r, VT* __restrict__ rs, uint32_t n, const CgScalars* sc)
 {
     const double inv = sc->rr > 0.0 ? 1.0 / sqrt(sc->rr) : 0.0;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) r32[i] = (float)(r[i] * inv);
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) rs[i] = (VT)(r[i] * inv);
 }
 
-// x += s * d32
-__global__ void correct_kernel(double* __restrict__ x, const float* __restrict__ d32, uint32_t n, const CgScalars* sc)
+// x += s * d
+template <typename VT>
+__global__ void correct_kernel(double* __restrict__ x, const VT* __restrict__ d, uint32_t n, const CgScalars* sc)
 {
     const double s = sqrt(sc->rr);
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) x[i] += s * (double)d32[i];
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) x[i] += s * (double)d[i];
 }
 
 __global__ void invert_kernel(const double* __restrict__ d, double* __restrict__ dinv, uint32_t n)
@@ -365,13 +367,25 @@ void residual(const double* b, const double* ax, double* r, uint32_t n, double* 
 
 void scale_to_f32(const double* r, float* r32, uint32_t n, const CgScalars* sc, hipStream_t s)
 {
-    scale_to_f32_kernel<<<reduction_grid(n), kBlock, 0, s>>>(r, r32, n, sc);
+    scale_kernel<float><<<reduction_grid(n), kBlock, 0, s>>>(r, r32, n, sc);
+    AMX_HIP(hipGetLastError());
+}
+
+void scale_to_f64(const double* r, double* rs, uint32_t n, const CgScalars* sc, hipStream_t s)
+{
+    scale_kernel<double><<<reduction_grid(n), kBlock, 0, s>>>(r, rs, n, sc);
     AMX_HIP(hipGetLastError());
 }
 
 void correct(double* x, const float* d32, uint32_t n, const CgScalars* sc, hipStream_t s)
 {
-    correct_kernel<<<reduction_grid(n), kBlock, 0, s>>>(x, d32, n, sc);
+    correct_kernel<float><<<reduction_grid(n), kBlock, 0, s>>>(x, d32, n, sc);
+    AMX_HIP(hipGetLastError());
+}
+
+void correct64(double* x, const double* d, uint32_t n, const CgScalars* sc, hipStream_t s)
+{
+    correct_kernel<double><<<reduction_grid(n), kBlock, 0, s>>>(x, d, n, sc);
     AMX_HIP(hipGetLastError());
 }
 

@@ -267,7 +267,10 @@ class ServingLoop:
         self.keep_x = keep_x
         self.clock = clock
 
-    def run(self, requests: list[Request], realtime: bool = True) -> dict:
+    def run(self, requests: list[Request], realtime: bool = True,
+            progress: Callable[[float, int, int, int], None] | None = None) -> dict:
+        """Serve `requests` (arrival times relative to the start); progress, if
+        given, is called as progress(elapsed_s, finished, running, pending)."""
         pending = collections.deque()
         incoming = collections.deque(sorted(requests, key=lambda r: r.arrival))
         running = {}  # job id -> (request, CachedOp, admitted time, workspace bytes)
@@ -302,6 +305,8 @@ class ServingLoop:
                 pending.popleft()
             peak_jobs = max(peak_jobs, len(running))
             peak_bytes = max(peak_bytes, workspace + self.cache.bytes)
+            if progress is not None:
+                progress(self.clock() - t0, len(done), len(running), len(pending) + len(incoming))
             if running:
                 self.server.step(self.n_iter)
                 for jid in self.server.poll():

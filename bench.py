@@ -219,9 +219,77 @@ def batch_mode(args) -> dict:
     return res
 
 
+def serve_mode(args) -> dict:
+    """Config C5: a stream of CG solve requests (60% hex8 Poisson n~U{16..96},
+    30% hex20 elasticity n~U{8..40}, 10% tet10 Poisson n~U{12..48}, own RHS,
+    1e-10 relative residual) arriving as Poisson traffic, served by the
+    continuous-batching server under an HBM budget; FP64 CG and FP64-refined
+    FP32/BF16 CG on the same request streams.  Operators are built once
+    (resident cache) before the timed streams."""
+    import numpy as np
+    from amat_amd import PREC, serve
+    rates = [float(r) for r in args.serve_rates.split(",")]
+    methods = {"fp64": PREC.FP64, "fp32": PREC.FP32, "bf16": PREC.BF16}
+    names = args.serve_methods.split(",")
+    streams = {r: serve.make_requests(args.serve_requests, r, seed=args.serve_seed) for r in rates}
+    keys = sorted({(q.kind, q.n) for st in streams.values() for q in st})
+    budget = int(args.serve_budget_gb * 2**30)
+    cache = serve.OperatorCache()
+    t0 = time.time()
+    for i, key in enumerate(keys):
+        cache.build(key)
+        if i % 10 == 0:
+            log(f"[serve] built {i + 1}/{len(keys)} operators ({cache.bytes / 2**30:.1f} GiB, {time.time() - t0:.0f}s)")
+    log(f"[serve] {len(keys)} operators resident: {cache.bytes / 2**30:.1f} GiB, built in {cache.build_seconds:.1f}s")
+    res = {"metric": "C5 served CG solves/s and latency at 1e-10 relative residual", "requests": args.serve_requests,
+           "rates": rates, "budget_GiB": args.serve_budget_gb, "slots": args.serve_slots, "step_iters": args.serve_iters,
+           "inner_rtol": args.inner_rtol,
+           "operators": len(keys), "operator_GiB": cache.bytes / 2**30, "operator_build_s": cache.build_seconds,
+           "runs": []}
+    for rate in rates:
+        for name in names:
+            server = serve.Server(max_jobs=args.serve_slots)
+            loop = serve.ServingLoop(server, cache, methods[name], budget, n_iter=args.serve_iters,
+                                     inner_rtol=args.inner_rtol)
+            last = [0.0]
+
+            def prog(el, fin, run, pend):
+                if el - last[0] >= 10.0:
+                    last[0] = el
+                    log(f"[serve] rate {rate:g} {name}: {el:.0f}s finished {fin} running {run} waiting {pend}")
+
+            m = loop.run(streams[rate], progress=prog)
+            done = m.pop("done")
+            by_kind = {}
+            for k in ("hex8", "hex20", "tet10"):
+                lat = [d.finished - d.arrival for d in done if d.kind == k]
+                if lat:
+                    by_kind[k] = {"count": len(lat), "p50_s": float(np.percentile(lat, 50)),
+                                  "p99_s": float(np.percentile(lat, 99))}
+            m.update({"rate": rate, "method": name, "by_kind": by_kind,
+                      "iterations": int(sum(d.iterations for d in done))})
+            res["runs"].append(m)
+            log(f"[serve] rate {rate:g} {name}: {m['solves_per_s']:.1f} solves/s, p50 {m['p50_s'] * 1e3:.0f} ms, "
+                f"p99 {m['p99_s'] * 1e3:.0f} ms, converged {m['all_converged']}, max relres {m['max_rel_residual']:.2e}")
+            del server
+    for rate in rates:
+        base = [r for r in res["runs"] if r["rate"] == rate and r["method"] == "fp64"]
+        for r in res["runs"]:
+            if base and r["rate"] == rate:
+                r["speedup_vs_fp64"] = r["solves_per_s"] / base[0]["solves_per_s"]
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--mode", choices=["matvec", "precision", "batch"], default="matvec")
+    ap.add_argument("--mode", choices=["matvec", "precision", "batch", "serve"], default="matvec")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-rates", default="50,200,800")
+    ap.add_argument("--serve-methods", default="fp64,bf16,fp32")
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-slots", type=int, default=64)
+    ap.add_argument("--serve-iters", type=int, default=8)
+    ap.add_argument("--serve-seed", type=int, default=7)
     ap.add_argument("--batch-n", type=int, default=48, help="hex20 elements per edge in --mode batch")
     ap.add_argument("--batch-nrhs", default="8,32,128")
     ap.add_argument("--batch-precisions", default="fp32,bf16")
@@ -252,6 +320,9 @@ def main() -> int:
         return 0
     if args.mode == "batch":
         print(json.dumps(batch_mode(args)), flush=True)
+        return 0
+    if args.mode == "serve":
+        print(json.dumps(serve_mode(args)), flush=True)
         return 0
     t0 = time.time()
     p = problems.elasticity_hex20(args.n)

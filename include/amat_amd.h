@@ -74,6 +74,9 @@ enum amx_matfree_type {
 
 /* storage precision of the element matrices (bit mask, amx_op_set_options) */
 enum amx_precision { AMX_PREC_FP64 = 1, AMX_PREC_FP32 = 2, AMX_PREC_BF16 = 4 };
+/* refinement option (or-ed into the precision argument of the refinement
+ * entry points): inner CG vectors in FP64 instead of FP32 */
+enum amx_ir_flags { AMX_IR_F64_VECTORS = 0x100 };
 
 /* how element vectors are summed into v */
 enum amx_scatter {
