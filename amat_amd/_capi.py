@@ -110,6 +110,7 @@ SIGNATURES = {
     "amx_mrhs_destroy": (I32, [P]),
     "amx_mrhs_set_tiles": (I32, [P, I32, I32, I32]),
     "amx_mrhs_get_tiles": (I32, [P, P, P, P]),
+    "amx_mrhs_set_scatter": (I32, [P, I32]),
     "amx_mrhs_apply_dev": (I32, [P, I32, P, P, P]),
     "amx_mrhs_autotune": (I32, [P, I32, P, P, I32, P, P]),
     "amx_mrhs_graph_capture": (I32, [P, I32, P, P, P]),

@@ -129,7 +129,10 @@ class Maps:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            _capi.lib().amx_maps_destroy(h)
+            try:
+                _capi.lib().amx_maps_destroy(h)
+            except Exception:  # interpreter shutdown
+                pass
             self._h = None
 
     @property
@@ -264,7 +267,10 @@ class aMatFree:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
-            _capi.lib().amx_op_destroy(h)
+            try:
+                _capi.lib().amx_op_destroy(h)
+            except Exception:  # interpreter shutdown
+                pass
             self._h = None
 
     @property
@@ -432,6 +438,9 @@ class MultiRHS:
 
     def set_tiles(self, rhs_tile: int, waves_per_block: int, work_split: int = 1):
         call("amx_mrhs_set_tiles", self._h, int(rhs_tile), int(waves_per_block), int(work_split))
+
+    def set_scatter(self, mode: SCATTER):
+        call("amx_mrhs_set_scatter", self._h, int(mode))
 
     def tiles(self) -> tuple[int, int, int]:
         a, b, c = C.c_int32(), C.c_int32(), C.c_int32()

@@ -13,7 +13,9 @@
 // per lane.  Smaller elements are packed block-diagonally, G = 32/n elements per
 // 32-row tile (4 hex8), from per-element matrices (FP32 column-major, BF16
 // row-major with 8-aligned rows); the off-diagonal zeros are never stored.
-// The element rows are added into V with FP32 atomics (order not fixed).
+// The element rows reach V either through the deterministic two-pass scatter
+// (rows stored per element, then summed per DoF in ascending element order:
+// bit-reproducible, independent of the launch shape) or FP32 atomics.
 // Work split: every segment (operator) is cut into 8 contiguous slices, one per
 // XCD (workgroups are dispatched round-robin over the 8 XCDs), so elements that
 // share nodes run on the same L2.
@@ -49,6 +51,7 @@ struct Seg {
     const uint32_t* dofs;   // [tile][dstride] local DoF ids (kNone: padding)
     const float* U;
     float* V;
+    float* ve;              // deterministic scatter: element rows [e*n + r][nrhs]; NULL: atomics into V
     uint32_t n, n_elem, n_tiles, ngroups, nrhs, dstride;
 };
 
@@ -106,8 +109,10 @@ __host__ __device__ inline uint32_t pos60(bool bf16, uint32_t r, uint32_t c)
 }
 
 // atomically add the C fragments acc[m][q] (rows of the tile) into V
+// (deterministic form: the rows go to their element rows of ve, row t*ROWS + tile row)
 template <int MT, int NT, int ROWS>
-__device__ __forceinline__ void scatter_rows(const Seg& sg, uint32_t mydof, uint32_t col0, const f32x16 (&acc)[MT][NT])
+__device__ __forceinline__ void scatter_rows(const Seg& sg, uint32_t t, uint32_t mydof, uint32_t col0,
+                                             const f32x16 (&acc)[MT][NT])
 {
     const uint32_t lane = threadIdx.x & 63u, half = lane >> 5, col_l = lane & 31u;
 #pragma unroll
@@ -121,7 +126,14 @@ __device__ __forceinline__ void scatter_rows(const Seg& sg, uint32_t mydof, uint
 #pragma unroll
             for (int q = 0; q < NT; q++) {
                 const uint32_t col = col0 + q * 32 + col_l;
-                if (col < sg.nrhs && d != kNone) unsafeAtomicAdd(&sg.V[(uint64_t)d * sg.nrhs + col], acc[m][q][i]);
+                if (col < sg.nrhs && d != kNone) {
+                    if (sg.ve) {
+                        const uint64_t row = (uint64_t)t * ROWS + (uint32_t)r0 + 4 * half;
+                        sg.ve[row * sg.nrhs + col] = acc[m][q][i];
+                    } else {
+                        unsafeAtomicAdd(&sg.V[(uint64_t)d * sg.nrhs + col], acc[m][q][i]);
+                    }
+                }
             }
         }
 }
@@ -267,8 +279,10 @@ __device__ __forceinline__ void mfma60(const Frag60<true>& A, const Rhs60<true, 
 
 // add the rows of a 60-DoF element into V: lane half h, C register i of M-tile
 // m is position 32 m + 8 (i/4) + 4 h + i%4 (padding positions 30, 31 skipped)
+// (deterministic form: element row 30 m + 8 (i/4) + 4 h + i%4 of element e in ve)
 template <int NT>
-__device__ __forceinline__ void scatter60(const Seg& sg, const uint32_t* sd, uint32_t col0, const f32x16 (&acc)[2][NT])
+__device__ __forceinline__ void scatter60(const Seg& sg, uint32_t e, const uint32_t* sd, uint32_t col0,
+                                          const f32x16 (&acc)[2][NT])
 {
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     const uint32_t lane = threadIdx.x & 63u, h = lane >> 5, col_l = lane & 31u;
@@ -289,7 +303,14 @@ __device__ __forceinline__ void scatter60(const Seg& sg, const uint32_t* sd, uin
 #pragma unroll
                 for (int q = 0; q < NT; q++) {
                     const uint32_t col = col0 + q * 32 + col_l;
-                    if (col < nrhs && !pad) unsafeAtomicAdd(V + __umul24(d[m][j][r], nrhs) + col, acc[m][q][4 * j + r]);
+                    if (col < nrhs && !pad) {
+                        if (sg.ve) {
+                            const uint64_t row = (uint64_t)e * 60 + 30 * m + 8 * j + 4 * h + r;
+                            sg.ve[row * nrhs + col] = acc[m][q][4 * j + r];
+                        } else {
+                            unsafeAtomicAdd(V + __umul24(d[m][j][r], nrhs) + col, acc[m][q][4 * j + r]);
+                        }
+                    }
                 }
             }
 }
@@ -379,7 +400,7 @@ __device__ __forceinline__ void tile_small(const Seg& sg, uint32_t t, uint32_t n
                                                                      __builtin_bit_cast(bf16x8, bw), acc[0][q], 0, 0, 0);
             }
     }
-    scatter_rows<1, NT, ROWS>(sg, mydof, col0, acc);
+    scatter_rows<1, NT, ROWS>(sg, t, mydof, col0, acc);
 }
 
 template <int N, bool BF16, int NT>
@@ -421,7 +442,7 @@ __device__ __forceinline__ void item60(const Seg& sg, uint32_t t, uint32_t ng, u
 #pragma unroll
             for (int i = 0; i < 16; i++) acc[m][q][i] = 0.0f;
     mfma60<NT>(A, B, acc);
-    scatter60<NT>(sg, sd, col0, acc);
+    scatter60<NT>(sg, t, sd, col0, acc);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -495,6 +516,20 @@ __global__ void zero_f32(float* p, uint64_t n)
     if (i < n) p[i] = 0.0f;
 }
 
+// deterministic scatter, second pass: V[d][c] = sum over d's element rows (fixed
+// order: ascending element, then row) of ve[row][c]
+__global__ __launch_bounds__(256) void mrhs_gather_kernel(const uint32_t* __restrict__ off,
+                                                         const uint32_t* __restrict__ pos, const float* __restrict__ ve,
+                                                         float* __restrict__ V, uint32_t n_total, uint32_t nrhs)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)n_total * nrhs) return;
+    const uint32_t d = (uint32_t)(i / nrhs), c = (uint32_t)(i - (uint64_t)d * nrhs);
+    float s = 0.0f;
+    for (uint32_t q = off[d], qe = off[d + 1]; q < qe; ++q) s += ve[(uint64_t)pos[q] * nrhs + c];
+    V[i] = s;
+}
+
 // element matrices for the MFMA kernels from the tiled FP64 storage:
 // 60 DoFs -> compact MFMA order (pos60); smaller -> FP32 column-major n x n or
 // BF16 row-major n x ld.  One thread per stored value.
@@ -539,9 +574,12 @@ struct amx_mrhs {
     bool bf16 = false;
     int rhs_tile = 32, wpb = 4;  // NT * 32 columns per wave, waves per block
     int chunked = 1;             // work split inside an XCD slice: 1 contiguous chunk per wave, 0 interleaved
+    int scatter = AMX_SCATTER_ATOMIC;
+    DevBuf ve;                   // deterministic scatter: element rows of all parts x nrhs
     struct Part {
         uint32_t n, n_elem, n_tiles, n_total;
-        DevBuf K, dofs;
+        uint64_t ve_row0;        // first element row of this part in ve
+        DevBuf K, dofs, off, pos;  // off/pos: DoF -> element rows (e*n + r), ascending
     };
     std::vector<Part> parts;
     hipGraph_t graph = nullptr;
@@ -577,9 +615,26 @@ void launch_persistent(amx_mrhs& m, const SegTable& tab, uint64_t max_slice, hip
     mrhs_persistent<B, T><<<dim3(nbx * kXcd), dim3(64 * m.wpb), 0, s>>>(tab);
 }
 
+// element-row buffer of the deterministic scatter (allocated outside launches,
+// so a captured graph never allocates)
+void prepare(amx_mrhs& m, int nrhs)
+{
+    if (m.scatter != AMX_SCATTER_DETERMINISTIC) return;
+    uint64_t rows = 0;
+    for (auto& p : m.parts) rows += (uint64_t)p.n_elem * p.n;
+    const uint64_t bytes = std::max<uint64_t>(rows * (uint64_t)nrhs * 4, 4);
+    if (m.ve.bytes < bytes) m.ve.alloc(bytes);
+}
+
 void launch(amx_mrhs& m, int nrhs, const float* const* U, float* const* V, hipStream_t s)
 {
     SegTable tab{};
+    const bool det = m.scatter == AMX_SCATTER_DETERMINISTIC;
+    if (det) {
+        uint64_t rows = 0;
+        for (auto& p : m.parts) rows += (uint64_t)p.n_elem * p.n;
+        AMX_REQUIRE(m.ve.bytes >= rows * (uint64_t)nrhs * 4, AMX_INVALID_STATE, "element-row buffer not prepared");
+    }
     AMX_REQUIRE(m.parts.size() <= (size_t)kMaxSeg, AMX_NOT_IMPLEMENTED, "too many element groups in one launch");
     const int NT = m.rhs_tile / 32;
     const uint32_t ngroups = (uint32_t)((nrhs + m.rhs_tile - 1) / m.rhs_tile);
@@ -600,9 +655,10 @@ void launch(amx_mrhs& m, int nrhs, const float* const* U, float* const* V, hipSt
         sg.ngroups = ngroups;
         sg.nrhs = (uint32_t)nrhs;
         sg.dstride = (uint32_t)dof_stride((int)p.n, m.bf16);
+        sg.ve = det ? m.ve.as<float>() + p.ve_row0 * (uint64_t)nrhs : nullptr;
         max_slice = std::max<uint64_t>(max_slice, ((uint64_t)p.n_tiles * ngroups + kXcd - 1) / kXcd);
         const uint64_t len = (uint64_t)p.n_total * nrhs;
-        if (len) zero_f32<<<(uint32_t)((len + 255) / 256), 256, 0, s>>>(V[i], len);
+        if (len && !det) zero_f32<<<(uint32_t)((len + 255) / 256), 256, 0, s>>>(V[i], len);
     }
     tab.n_seg = (uint32_t)m.parts.size();
     tab.chunked = (uint32_t)m.chunked;
@@ -617,6 +673,17 @@ void launch(amx_mrhs& m, int nrhs, const float* const* U, float* const* V, hipSt
         else launch_persistent<false, 4>(m, tab, max_slice, s);
     }
     AMX_HIP(hipGetLastError());
+    if (det) {
+        for (size_t i = 0; i < m.parts.size(); i++) {
+            auto& p = m.parts[i];
+            const uint64_t len = (uint64_t)p.n_total * nrhs;
+            if (!len) continue;
+            mrhs_gather_kernel<<<(uint32_t)((len + 255) / 256), 256, 0, s>>>(
+                p.off.as<uint32_t>(), p.pos.as<uint32_t>(), m.ve.as<float>() + p.ve_row0 * (uint64_t)nrhs, V[i],
+                p.n_total, (uint32_t)nrhs);
+        }
+        AMX_HIP(hipGetLastError());
+    }
 }
 
 }  // namespace
@@ -672,6 +739,27 @@ int amx_mrhs_create(amx_op* const* ops, int n_ops, int precision, amx_mrhs** out
                     dofs[(size_t)t * DS + k] = op.local_map[op.eoff[e] + k % g.n];
                 }
             }
+            // DoF -> element rows (e * n + r), ascending: the deterministic scatter's lists
+            {
+                p.ve_row0 = 0;
+                for (auto& q : m->parts) p.ve_row0 += (uint64_t)q.n_elem * q.n;
+                std::vector<uint32_t> off(op.n_total + 1, 0), pos((size_t)p.n_elem * g.n);
+                for (uint32_t j = 0; j < p.n_elem; j++) {
+                    const uint32_t* em = &op.local_map[op.eoff[g.elems[g.slot_elem[slot_of[j]]]]];
+                    for (uint32_t r = 0; r < g.n; r++) off[em[r] + 1]++;
+                }
+                for (uint32_t d = 0; d < op.n_total; d++) off[d + 1] += off[d];
+                std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+                for (uint32_t j = 0; j < p.n_elem; j++) {
+                    const uint32_t* em = &op.local_map[op.eoff[g.elems[g.slot_elem[slot_of[j]]]]];
+                    for (uint32_t r = 0; r < g.n; r++) pos[fill[em[r]]++] = j * g.n + r;
+                }
+                AMX_REQUIRE((uint64_t)p.n_elem * g.n < (1ull << 32), AMX_NOT_IMPLEMENTED, "too many element rows");
+                p.off.alloc(off.size() * 4);
+                AMX_HIP(hipMemcpy(p.off.p, off.data(), off.size() * 4, hipMemcpyHostToDevice));
+                p.pos.alloc(std::max<size_t>(pos.size(), 1) * 4);
+                AMX_HIP(hipMemcpy(p.pos.p, pos.data(), pos.size() * 4, hipMemcpyHostToDevice));
+            }
             DevBuf dslot;
             dslot.alloc(std::max<size_t>(slot_of.size(), 1) * 4);
             AMX_HIP(hipMemcpy(dslot.p, slot_of.data(), slot_of.size() * 4, hipMemcpyHostToDevice));
@@ -718,6 +806,16 @@ int amx_mrhs_set_tiles(amx_mrhs* m, int rhs_tile, int waves_per_block, int work_
     });
 }
 
+int amx_mrhs_set_scatter(amx_mrhs* m, int scatter)
+{
+    return guarded(__func__, [&] {
+        AMX_REQUIRE(m, AMX_INVALID_ARGUMENT, "NULL handle");
+        AMX_REQUIRE(scatter == AMX_SCATTER_DETERMINISTIC || scatter == AMX_SCATTER_ATOMIC, AMX_INVALID_ARGUMENT,
+                    "scatter: AMX_SCATTER_DETERMINISTIC or AMX_SCATTER_ATOMIC");
+        m->scatter = scatter;
+    });
+}
+
 int amx_mrhs_get_tiles(const amx_mrhs* m, int* rhs_tile, int* waves_per_block, int* work_split)
 {
     return guarded(__func__, [&] {
@@ -733,6 +831,7 @@ int amx_mrhs_apply_dev(amx_mrhs* m, int nrhs, const float* const* U, float* cons
     return guarded(__func__, [&] {
         AMX_REQUIRE(m && U && V && nrhs > 0, AMX_INVALID_ARGUMENT, "bad argument");
         amx::DeviceGuard dg(m->device);
+        amx::prepare(*m, nrhs);
         amx::launch(*m, nrhs, U, V, static_cast<hipStream_t>(stream));
     });
 }
@@ -744,6 +843,7 @@ int amx_mrhs_autotune(amx_mrhs* m, int nrhs, const float* const* U, float* const
         AMX_REQUIRE(m && U && V && nrhs > 0 && reps > 0, AMX_INVALID_ARGUMENT, "bad argument");
         amx::DeviceGuard dg(m->device);
         hipStream_t s = static_cast<hipStream_t>(stream);
+        amx::prepare(*m, nrhs);
         hipEvent_t e0, e1;
         AMX_HIP(hipEventCreate(&e0));
         AMX_HIP(hipEventCreate(&e1));
@@ -792,6 +892,7 @@ int amx_mrhs_graph_capture(amx_mrhs* m, int nrhs, const float* const* U, float* 
         AMX_REQUIRE(stream != nullptr, AMX_INVALID_ARGUMENT, "capture needs a non-default stream");
         amx::DeviceGuard dg(m->device);
         hipStream_t s = static_cast<hipStream_t>(stream);
+        amx::prepare(*m, nrhs);
         if (m->exec) { (void)hipGraphExecDestroy(m->exec); m->exec = nullptr; }
         if (m->graph) { (void)hipGraphDestroy(m->graph); m->graph = nullptr; }
         AMX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));

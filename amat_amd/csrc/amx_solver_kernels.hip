@@ -33,6 +33,14 @@ __device__ __forceinline__ double block_sum(double x, double* sh)
 // Deposit this block's NV partial sums; return true in the block that must
 // finish the reduction (all partials visible to it), with the totals in tot.
 // blk / nblk: this block's index among the reduction's blocks and their count.
+//
+// No device-scope fence: on this chip a release/acquire fence at agent scope
+// writes back / invalidates the whole L2 of the XCD (buffer_wbl2 / buffer_inv),
+// which, once per block, dominated the solver kernels.  Instead the partials
+// are deposited with agent-scope atomic exchanges (performed at the point of
+// coherence of all XCDs) whose completion is awaited before the counter is
+// incremented, and the finishing block reads them with agent-scope atomic
+// loads.  Summation order is unchanged (fixed, by block index).
 template <int NV>
 __device__ bool reduce_partials_n(const double (&v)[NV], double* partials, uint32_t* counter, double (&tot)[NV],
                                   double* sh, uint32_t blk, uint32_t nblk)
@@ -41,21 +49,25 @@ __device__ bool reduce_partials_n(const double (&v)[NV], double* partials, uint3
     double b[NV];
     for (int k = 0; k < NV; k++) b[k] = block_sum(v[k], sh);
     if (threadIdx.x == 0) {
-        for (int k = 0; k < NV; k++) partials[(size_t)k * nblk + blk] = b[k];
-        __threadfence();
-        const uint32_t prev = atomicAdd(counter, 1u);
+        double old = 0.0;
+        for (int k = 0; k < NV; k++)
+            old += __hip_atomic_exchange(&partials[(size_t)k * nblk + blk], b[k], __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+        // every exchange has returned (its value is used) before the counter moves
+        const uint32_t one = old == old ? 1u : 1u;
+        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        const uint32_t prev = __hip_atomic_fetch_add(counter, one, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         last = (prev == nblk - 1);
     }
     __syncthreads();
     if (!last) return false;
-    __threadfence();
     for (int k = 0; k < NV; k++) {
         double s = 0.0;
         for (uint32_t i = threadIdx.x; i < nblk; i += kBlock)
             s += __hip_atomic_load(&partials[(size_t)k * nblk + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         tot[k] = block_sum(s, sh);
     }
-    if (threadIdx.x == 0) *counter = 0u;
+    if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return true;
 }
 

@@ -155,7 +155,7 @@ def batch_mode(args) -> dict:
     operator."""
     import numpy as np
     import torch
-    from amat_amd import PREC, Maps, MultiRHS, aMatFree, mesh, problems
+    from amat_amd import PREC, SCATTER, Maps, MultiRHS, aMatFree, mesh, problems
     t0 = time.time()
     probs = [problems.elasticity_hex20(args.batch_n), problems.poisson_hex8(80, 40, 20)]
     ops = [p.build_operator(aMatFree, Maps, precision=PREC.FP64) for p in probs]
@@ -165,56 +165,59 @@ def batch_mode(args) -> dict:
     res = {"metric": "C4 multi-RHS EbE products (hex20 48^3 + hex8 80x40x20, one persistent launch)",
            "operators": [{"name": p.name, "elements": p.n_elem, "dofs": p.n_dofs, "dofs_per_element": p.dofs_per_elem}
                          for p in probs], "cases": []}
-    for prec_name in args.batch_precisions.split(","):
+    scatters = {"atomic": SCATTER.ATOMIC, "deterministic": SCATTER.DETERMINISTIC}
+    for prec_name, sc_name, nrhs in [(p_, s_, int(n_)) for p_ in args.batch_precisions.split(",")
+                                     for s_ in args.batch_scatter.split(",") for n_ in args.batch_nrhs.split(",")]:
         prec = PREC.FP32 if prec_name == "fp32" else PREC.BF16
         mr = MultiRHS(ops, prec)
+        mr.set_scatter(scatters[sc_name])
         sk = 4 if prec == PREC.FP32 else 2
-        for nrhs in [int(x) for x in args.batch_nrhs.split(",")]:
-            U = [torch.from_numpy(mesh.uniform(21 + i, 7, p.n_dofs * nrhs, -1.0, 1.0).astype(np.float32)
-                                  .reshape(p.n_dofs, nrhs)).cuda() for i, p in enumerate(probs)]
-            V = [torch.empty_like(u) for u in U]
-            with torch.cuda.stream(s):
-                tunes = mr.autotune(nrhs, U, V, reps=5, stream=s)
-                for _ in range(args.warmup):
-                    mr.apply_dev(nrhs, U, V, s)
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                s.synchronize()
-                e0.record(s)
-                for _ in range(args.steps):
-                    mr.apply_dev(nrhs, U, V, s)
-                e1.record(s)
-                s.synchronize()
-                ms = e0.elapsed_time(e1) / args.steps
-                mr.graph_capture(nrhs, U, V, s)
+        U = [torch.from_numpy(mesh.uniform(21 + i, 7, p.n_dofs * nrhs, -1.0, 1.0).astype(np.float32)
+                              .reshape(p.n_dofs, nrhs)).cuda() for i, p in enumerate(probs)]
+        V = [torch.empty_like(u) for u in U]
+        with torch.cuda.stream(s):
+            tunes = mr.autotune(nrhs, U, V, reps=5, stream=s)
+            for _ in range(args.warmup):
+                mr.apply_dev(nrhs, U, V, s)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            s.synchronize()
+            e0.record(s)
+            for _ in range(args.steps):
+                mr.apply_dev(nrhs, U, V, s)
+            e1.record(s)
+            s.synchronize()
+            ms = e0.elapsed_time(e1) / args.steps
+            mr.graph_capture(nrhs, U, V, s)
+            mr.graph_launch(s)
+            s.synchronize()
+            e0.record(s)
+            for _ in range(args.steps):
                 mr.graph_launch(s)
-                s.synchronize()
-                e0.record(s)
-                for _ in range(args.steps):
-                    mr.graph_launch(s)
-                e1.record(s)
-                s.synchronize()
-                ms_graph = e0.elapsed_time(e1) / args.steps
-            # column 0 against the FP64 operator (plain element sum)
-            errs = []
-            for i, (p, op) in enumerate(zip(probs, ops)):
-                u0 = U[i][:, 0].double().contiguous()
-                v0 = torch.empty_like(u0)
-                op.matvec_dev(v0, u0, torch.cuda.current_stream())
-                errs.append(float(torch.linalg.norm(V[i][:, 0].double() - v0) / torch.linalg.norm(v0)))
-            alg = sum(p.n_elem * (p.dofs_per_elem ** 2 * sk + 4 * p.dofs_per_elem) + p.n_dofs * 8 * nrhs
-                      for p in probs)
-            flops = sum(2 * p.n_elem * p.dofs_per_elem ** 2 * nrhs for p in probs)
-            best = min(ms, ms_graph)
-            case = {"precision": prec_name, "nrhs": nrhs, "tile": list(mr.tiles()), "ms": ms, "ms_graph": ms_graph,
-                    "GDoF_rhs_per_s": sum(p.n_dofs for p in probs) * nrhs / (best * 1e-3) / 1e9,
-                    "alg_bytes": alg, "hbm_frac": alg / (best * 1e-3) / HBM_PEAK,
-                    "tflops": flops / (best * 1e-3) / 1e12, "mfma_frac": flops / (best * 1e-3) / MFMA_PEAK[prec_name],
-                    "col0_rel_err_vs_fp64": errs,
-                    "autotune_ms": {f"{rt}x{w}x{c}": round(t, 4) for (rt, w, c), t in tunes.items()}}
-            res["cases"].append(case)
-            log(f"[batch] {prec_name} nrhs {nrhs}: tile {mr.tiles()} {ms:.4f} ms (graph {ms_graph:.4f}), "
-                f"{case['hbm_frac'] * 100:.1f}% HBM, {case['tflops']:.1f} TFLOP/s, err {errs}")
-            del U, V
+            e1.record(s)
+            s.synchronize()
+            ms_graph = e0.elapsed_time(e1) / args.steps
+        # column 0 against the FP64 operator (plain element sum)
+        errs = []
+        for i, (p, op) in enumerate(zip(probs, ops)):
+            u0 = U[i][:, 0].double().contiguous()
+            v0 = torch.empty_like(u0)
+            op.matvec_dev(v0, u0, torch.cuda.current_stream())
+            errs.append(float(torch.linalg.norm(V[i][:, 0].double() - v0) / torch.linalg.norm(v0)))
+        alg = sum(p.n_elem * (p.dofs_per_elem ** 2 * sk + 4 * p.dofs_per_elem) + p.n_dofs * 8 * nrhs
+                  for p in probs)
+        flops = sum(2 * p.n_elem * p.dofs_per_elem ** 2 * nrhs for p in probs)
+        best = min(ms, ms_graph)
+        case = {"precision": prec_name, "scatter": sc_name, "nrhs": nrhs, "tile": list(mr.tiles()), "ms": ms,
+                "ms_graph": ms_graph,
+                "GDoF_rhs_per_s": sum(p.n_dofs for p in probs) * nrhs / (best * 1e-3) / 1e9,
+                "alg_bytes": alg, "hbm_frac": alg / (best * 1e-3) / HBM_PEAK,
+                "tflops": flops / (best * 1e-3) / 1e12, "mfma_frac": flops / (best * 1e-3) / MFMA_PEAK[prec_name],
+                "col0_rel_err_vs_fp64": errs,
+                "autotune_ms": {f"{rt}x{w}x{c}": round(t, 4) for (rt, w, c), t in tunes.items()}}
+        res["cases"].append(case)
+        log(f"[batch] {prec_name} {sc_name} nrhs {nrhs}: tile {mr.tiles()} {ms:.4f} ms (graph {ms_graph:.4f}), "
+            f"{case['hbm_frac'] * 100:.1f}% HBM, {case['tflops']:.1f} TFLOP/s, err {errs}")
+        del U, V
         del mr
     return res
 
@@ -293,6 +296,7 @@ def main() -> int:
     ap.add_argument("--batch-n", type=int, default=48, help="hex20 elements per edge in --mode batch")
     ap.add_argument("--batch-nrhs", default="8,32,128")
     ap.add_argument("--batch-precisions", default="fp32,bf16")
+    ap.add_argument("--batch-scatter", default="atomic,deterministic")
     ap.add_argument("--inner-rtol", type=float, default=1e-2)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)

@@ -325,8 +325,10 @@ int amx_ir_cg_solve_dev(amx_op* op, int precision, const double* b, double* x, d
  * a batched GEMV per colour, include/aMatGpu.hpp:21-591); this is the same
  * product for nrhs vectors at once, on MFMA: FP32 storage with FP32 MFMA, or
  * BF16 storage and BF16-rounded U with FP32 accumulation.  No boundary
- * conditions (the plain element sum of aMatFree::matvec); the element rows are
- * added with FP32 atomics, so the summation order is not fixed.
+ * conditions (the plain element sum of aMatFree::matvec).  The element rows are
+ * summed into V with FP32 atomics (default, order not fixed) or
+ * deterministically (rows stored, then summed per DoF in a fixed order:
+ * bit-reproducible, independent of the launch shape), amx_mrhs_set_scatter.
  * Element types: 8, 10, 20 and 60 DoFs per element, one type per operator. */
 typedef struct amx_mrhs amx_mrhs;
 /* Copies the operators' element matrices (from their FP64 storage) into the
@@ -339,6 +341,8 @@ int amx_mrhs_destroy(amx_mrhs* m);
  * run of element tiles of its XCD's share, 0: the waves interleave over it. */
 int amx_mrhs_set_tiles(amx_mrhs* m, int rhs_tile, int waves_per_block, int work_split);
 int amx_mrhs_get_tiles(const amx_mrhs* m, int* rhs_tile, int* waves_per_block, int* work_split);
+/* AMX_SCATTER_ATOMIC (default) or AMX_SCATTER_DETERMINISTIC. */
+int amx_mrhs_set_scatter(amx_mrhs* m, int scatter);
 /* U[i], V[i]: device arrays of operator i (n_ops of each).  V is overwritten. */
 int amx_mrhs_apply_dev(amx_mrhs* m, int nrhs, const float* const* U, float* const* V, void* stream);
 /* Times every launch shape (reps launches each) on the given arrays, keeps

@@ -7,14 +7,15 @@ oracle's FP64 matvec of column i.  Two bars:
     also U rounded to BF16), so only FP32 accumulation differs: rel L2 <= 2e-6
     per column;
   * against the FP64 operator: FP32 storage <= 1e-6, BF16 storage <= 1e-2.
-The atomic scatter makes the FP32 sums order-dependent, hence tolerances and
-no bit-exactness.
+The atomic scatter (default) makes the FP32 sums order-dependent (tolerances
+only); the deterministic scatter sums element rows per DoF in a fixed order:
+its results are bit-identical run to run and across launch shapes.
 """
 import numpy as np
 import pytest
 
 import oracle as orc
-from amat_amd import PREC, AmatError, Maps, MultiRHS, aMatFree, mesh, problems
+from amat_amd import PREC, SCATTER, AmatError, Maps, MultiRHS, aMatFree, mesh, problems
 from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -77,16 +78,37 @@ def check(p, precision, U, V):
     return worst_model, worst_64
 
 
+@pytest.mark.parametrize("scatter", [SCATTER.DETERMINISTIC, SCATTER.ATOMIC])
 @pytest.mark.parametrize("precision", [PREC.FP32, PREC.BF16])
 @pytest.mark.parametrize("case", list(CASES))
 @pytest.mark.parametrize("nrhs", [1, 8, 37])
-def test_single_operator(torch_gpu, case, precision, nrhs):
+def test_single_operator(torch_gpu, case, precision, nrhs, scatter):
     torch = torch_gpu
     p = CASES[case]()
     ops, mr = make(torch, [p], precision)
+    mr.set_scatter(scatter)
     U, dU, dV = inputs(torch, [p], nrhs)
     mr.apply_dev(nrhs, dU, dV, torch.cuda.current_stream())
     check(p, precision, U[0], dV[0].cpu().numpy())
+
+
+@pytest.mark.parametrize("precision", [PREC.FP32, PREC.BF16])
+def test_deterministic_bitwise_across_runs_and_shapes(torch_gpu, precision):
+    torch = torch_gpu
+    probs = [CASES["hex20_elast"](), CASES["hex8"](), CASES["tet10"]()]
+    ops, mr = make(torch, probs, precision)
+    mr.set_scatter(SCATTER.DETERMINISTIC)
+    nrhs = 40
+    U, dU, dV = inputs(torch, probs, nrhs)
+    mr.apply_dev(nrhs, dU, dV, torch.cuda.current_stream())
+    first = [v.cpu().numpy().copy() for v in dV]
+    for rt, w, c in MultiRHS.TILE_SHAPES:
+        mr.set_tiles(rt, w, c)
+        for v in dV:
+            v.fill_(float("nan"))
+        mr.apply_dev(nrhs, dU, dV, torch.cuda.current_stream())
+        for a, v in zip(first, dV):
+            np.testing.assert_array_equal(v.cpu().numpy(), a)
 
 
 @pytest.mark.parametrize("precision", [PREC.FP32, PREC.BF16])
