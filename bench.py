@@ -34,9 +34,10 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(p, n_sample_layers: int, reps: int, threads: int) -> dict:
+def cpu_baseline(p, n_sample_layers: int, seconds: float, threads: int) -> dict:
     """The oracle's matvec_ghosted_OMP (default build: row-major scalar GEMV +
-    omp atomic) on the first n_sample_layers element layers of the workload,
+    omp atomic) on the first n_sample_layers element layers of the workload
+    (all of them by default), repeated for about `seconds` of CPU time and
     scaled to the whole mesh (time per element x all elements)."""
     import numpy as np
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -47,16 +48,19 @@ def cpu_baseline(p, n_sample_layers: int, reps: int, threads: int) -> dict:
     gm = p.gmap[:ns]
     K = p.element_matrices(0, ns)
     u = p.random_u()
-    orc.matvec(gm, K, u, n_total=p.n_dofs, nthreads=threads)  # warm-up
+    t0 = time.perf_counter()
+    orc.matvec(gm, K, u, n_total=p.n_dofs, nthreads=threads)  # warm-up, and the per-matvec estimate
+    t1 = time.perf_counter() - t0
+    reps = max(1, int(seconds / max(t1, 1e-6)))
     t0 = time.perf_counter()
     for _ in range(reps):
         orc.matvec(gm, K, u, n_total=p.n_dofs, nthreads=threads)
     dt = (time.perf_counter() - t0) / reps
     t_full = dt * E / ns
     return {"value": p.n_dofs / t_full / 1e9, "unit": "GDoF/s", "cores": threads, "kind": "port",
-            "sample": f"{ns} of {E} elements ({n_sample_layers} element layers), {reps} matvecs, "
-                      f"{dt * 1e3:.1f} ms each, scaled by element count; oracle matvec_ghosted_OMP "
-                      f"(row-major scalar GEMV + omp atomic), {threads} OpenMP threads"}
+            "sample": f"{ns} of {E} elements ({ns // per_layer} element layers), {reps} matvecs "
+                      f"({dt * reps:.1f} s), {dt * 1e3:.1f} ms each, scaled by element count; oracle "
+                      f"matvec_ghosted_OMP (row-major scalar GEMV + omp atomic), {threads} OpenMP threads"}
 
 
 def load_traffic(name: str):
@@ -303,8 +307,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=64, help="elements per cube edge")
     ap.add_argument("--scatter", choices=["deterministic", "atomic"], default="deterministic")
-    ap.add_argument("--cpu-layers", type=int, default=4, help="element layers in the CPU baseline sample")
-    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--cpu-layers", type=int, default=1 << 30, help="element layers in the CPU baseline sample (all)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU time of the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed region, copy 1 GiB device-to-device 5 times (known bytes for the "
@@ -389,7 +393,7 @@ def main() -> int:
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         log(f"[bench] CPU baseline on {threads} threads ...")
-        out["cpu_baseline"] = cpu_baseline(p, args.cpu_layers, args.cpu_reps, threads)
+        out["cpu_baseline"] = cpu_baseline(p, args.cpu_layers, args.cpu_seconds, threads)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
