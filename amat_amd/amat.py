@@ -68,6 +68,13 @@ class PREC(enum.IntFlag):
 IR_F64_VECTORS = 0x100
 
 
+class ELEM(enum.IntEnum):
+    """Element kinds of method 2 (FREE), computed in-kernel from coordinates."""
+    HEX8_LAPLACE = 1
+    HEX20_ELASTIC = 2
+    TET10_LAPLACE = 3
+
+
 class SCATTER(enum.IntEnum):
     DETERMINISTIC = 0
     ATOMIC = 1
@@ -255,7 +262,8 @@ class Maps:
 
 
 class aMatFree:
-    """par::aMatFree (include/aMatFree.hpp) running MATFREE_TYPE 3/4/5 on one GPU."""
+    """par::aMatFree (include/aMatFree.hpp) running MATFREE_TYPE 2 (built-in element
+    kinds) and 3/4/5 on one GPU."""
 
     def __init__(self, mesh_maps: Maps, bcType: BC_METH = BC_METH.BC_IMATRIX, device: int = 0):
         h = C.c_void_p()
@@ -301,6 +309,14 @@ class aMatFree:
         kr = _arr(k_ref, np.float64)
         sc = _arr(scale, np.float64)
         call("amx_op_set_element_matrices_scaled", self._h, _ptr(kr), 0, _ptr(sc))
+
+    def set_element_function(self, kind: ELEM, xe, scale=None, E: float = 1.0, nu: float = 0.3, ngt: int = 2):
+        """aMat::set_element_matrix_function (include/aMat.hpp:114-116) for method 2
+        (FREE): xe is (n_elems, nodes, 3) node coordinates in element DoF-node
+        order; K_e = scale[e] * ke(xe[e]) is applied in-kernel at every matvec."""
+        x = _arr(xe, np.float64)
+        sc = None if scale is None else _arr(scale, np.float64)
+        call("amx_op_set_element_function", self._h, int(kind), _ptr(x), _ptr(sc), float(E), float(nu), int(ngt))
 
     def set_options(self, precision=PREC.FP64, scatter=SCATTER.DETERMINISTIC):
         call("amx_op_set_options", self._h, int(precision), int(scatter))

@@ -37,6 +37,7 @@ SOURCES = [
     "amx_kernels.hip",
     "amx_solver_kernels.hip",
     "amx_mrhs.hip",
+    "amx_free.hip",
 ]
 
 COMMON = [

@@ -8,6 +8,7 @@
 // examples/src/ke_matrix.cpp, examples/include/integration.hpp) so that the
 // same coordinates give the same matrices (checked against fixtures written by
 // the reference sources to ~1e-15 relative).
+#include <algorithm>
 #include <cmath>
 
 #include "amx_internal.hpp"
@@ -270,6 +271,51 @@ void check_ngt(uint32_t ngt)
 }
 
 }  // namespace
+
+namespace amx {
+
+// Quadrature table of the FREE kernels: per point, dN_a/dxi_d at 3a+d and the
+// weight at 3*nodes; the points and their order are those of ke_hex8 /
+// ke_hex20_iso (i, j, k loops, k fastest) and of the tetrahedral rule of
+// ke_tet10.  Rows from nq up to `rows` repeat the first point with weight 0.
+std::vector<double> free_quadrature(int kind, uint32_t ngt, uint32_t& nq, uint32_t rows_min)
+{
+    check_ngt(ngt);
+    const int nn = kind == AMX_ELEM_HEX8_LAPLACE ? 8 : (kind == AMX_ELEM_HEX20_ELASTIC ? 20 : 10);
+    const uint32_t w = 3 * nn + 1;
+    std::vector<double> t;
+    if (kind == AMX_ELEM_TET10_LAPLACE) {
+        tet_rule(ngt, [&](const double* x, double cm) {
+            const size_t o = t.size();
+            t.resize(o + w);
+            shape_tet10(x, nullptr, t.data() + o);
+            t[o + 3 * nn] = cm;
+        });
+    } else {
+        double pw[2 * 16];
+        gauss_legendre(ngt, pw);
+        for (uint32_t i = 0; i < ngt; i++)
+            for (uint32_t j = 0; j < ngt; j++)
+                for (uint32_t k = 0; k < ngt; k++) {
+                    const double xi[3] = {pw[2 * i], pw[2 * j], pw[2 * k]};
+                    const size_t o = t.size();
+                    t.resize(o + w);
+                    if (nn == 8) dshape_hex8(xi, t.data() + o);
+                    else dshape_hex20(xi, t.data() + o);
+                    t[o + 3 * nn] = pw[2 * k + 1] * pw[2 * j + 1] * pw[2 * i + 1];
+                }
+    }
+    nq = (uint32_t)(t.size() / w);
+    for (uint32_t r = nq; r < rows_min; r++) {
+        const size_t o = t.size();
+        t.resize(o + w);
+        std::copy(t.begin(), t.begin() + w, t.begin() + o);
+        t[o + 3 * nn] = 0.0;
+    }
+    return t;
+}
+
+}  // namespace amx
 
 extern "C" {
 

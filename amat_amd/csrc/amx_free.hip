@@ -1,0 +1,393 @@
+// amx_free.hip -- method 2 (FREE): element contributions computed from node
+// coordinates at every application; no element matrices are stored.
+//
+// The reference's FREE method calls a user function that builds K_e from the
+// element's coordinates before each element GEMV (aMat::set_element_matrix_
+// function, include/aMat.hpp:114-116, called from aMatFree.hpp:2449-2476;
+// the examples pass ke_hex20_iso through computeElemMat,
+// examples/src/example07a.cpp:54-70).  Here the element kinds of the examples
+// are built in and applied without forming K_e, with the quadrature, shape
+// functions and constitutive matrix of ke_hex8 / ke_tet10 / ke_hex20_iso
+// (examples/src/ke_matrix.cpp), so f = s_e K_e u_e up to rounding:
+//
+//   J[k][d] = sum_a x_ak dN_ad,  inv = J^-1 (inv[d][k] = dxi_d/dx_k),
+//   w = |J| w_q s_e,  dN_ad = dN_a/dxi_d at the quadrature point.
+//   Laplace (one DoF per node):
+//     h_d = sum_a u_a dN_ad,  p_d = w sum_k inv[d][k] (sum_d' h_d' inv[d'][k]),
+//     f_a += sum_d dN_ad p_d.
+//   Isotropic elasticity (DoF 3a+i, Voigt xx,yy,zz,xy,yz,xz, engineering shear):
+//     H_id = sum_a u_ai dN_ad,  grad u = H inv,  sigma = w C eps(u),
+//     P_di = sum_k inv[d][k] sigma_ik,  f_ai += sum_d dN_ad P_di.
+//
+// Outputs go to the element rows of the deterministic scatter (or atomics), so
+// the maps, BC forms and gather are those of the stored-matrix kernels.  With
+// DIAG the kernels write diag(s_e K_e) instead (Jacobi, penalty, trace).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "amx_internal.hpp"
+#include "amx_kernels.hpp"
+
+namespace amx {
+
+namespace {
+
+constexpr uint32_t kFlag = 0x80000000u, kMask = 0x7fffffffu;
+
+__device__ __forceinline__ void lds_sync_wave()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// det J and inv[d][k] = cofactor(J)[k][d] / det (cofactors as in the host
+// element routines, amx_fem.cpp jacobian; one division)
+__device__ __forceinline__ double jac_inv_r(const double J[3][3], double inv[3][3])
+{
+    const double c00 = J[1][1] * J[2][2] - J[2][1] * J[1][2];
+    const double c10 = J[2][1] * J[0][2] - J[0][1] * J[2][2];
+    const double c20 = J[0][1] * J[1][2] - J[1][1] * J[0][2];
+    const double det = J[0][0] * c00 + J[1][0] * c10 + J[2][0] * c20;
+    const double r = 1.0 / det;
+    inv[0][0] = c00 * r;
+    inv[0][1] = c10 * r;
+    inv[0][2] = c20 * r;
+    inv[1][0] = (J[1][2] * J[2][0] - J[2][2] * J[1][0]) * r;
+    inv[1][1] = (J[2][2] * J[0][0] - J[0][2] * J[2][0]) * r;
+    inv[1][2] = (J[0][2] * J[1][0] - J[1][2] * J[0][0]) * r;
+    inv[2][0] = (J[2][1] * J[1][0] - J[1][1] * J[2][0]) * r;
+    inv[2][1] = (J[0][1] * J[2][0] - J[2][1] * J[0][0]) * r;
+    inv[2][2] = (J[1][1] * J[0][0] - J[0][1] * J[1][0]) * r;
+    return det;
+}
+
+// ---- Laplace elements: one lane per element slot ------------------------
+// xe: [NN*3][stride] (coordinate-major, slot fastest), qp: [nq][NN*3+1]
+// (dN_ad at 3a+d, weight last; uniform across lanes -> scalar loads).
+template <int NN, bool MASKED, bool ATOMIC, bool DIAG>
+__global__ __launch_bounds__(256) void free_laplace_kernel(const double* __restrict__ xe, const double* __restrict__ scale,
+                                                          const uint32_t* __restrict__ map, const double* __restrict__ u,
+                                                          double* __restrict__ out, uint64_t ve_base, uint32_t slot_begin,
+                                                          uint32_t n_slots, uint32_t stride, const double* __restrict__ qp,
+                                                          uint32_t nq, const uint32_t* __restrict__ skip)
+{
+    if (skip && *skip) return;
+    const uint32_t sl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sl >= n_slots) return;
+    const uint64_t slot = (uint64_t)slot_begin + sl;
+    double x[NN * 3], ue[NN], f[NN];
+    uint32_t m[NN];
+#pragma unroll
+    for (int i = 0; i < NN * 3; i++) x[i] = xe[(uint64_t)i * stride + slot];
+#pragma unroll
+    for (int a = 0; a < NN; a++) {
+        m[a] = map[slot * NN + a];
+        ue[a] = 0.0;
+        if constexpr (!DIAG) {
+            const double v = u[m[a] & kMask];
+            ue[a] = (MASKED && (m[a] & kFlag)) ? 0.0 : v;
+        }
+        f[a] = 0.0;
+    }
+    const double sc = scale[slot];
+    for (uint32_t q = 0; q < nq; q++) {
+        const double* t = qp + (uint64_t)q * (NN * 3 + 1);
+        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, h[3] = {0, 0, 0};
+#pragma unroll
+        for (int a = 0; a < NN; a++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double dn = t[3 * a + d];
+#pragma unroll
+                for (int k = 0; k < 3; k++) J[k][d] += x[3 * a + k] * dn;
+                if constexpr (!DIAG) h[d] += ue[a] * dn;
+            }
+        double inv[3][3];
+        const double w = jac_inv_r(J, inv) * t[NN * 3] * sc;
+        if constexpr (DIAG) {
+#pragma unroll
+            for (int a = 0; a < NN; a++) {
+                double s = 0.0;
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const double g = t[3 * a] * inv[0][k] + t[3 * a + 1] * inv[1][k] + t[3 * a + 2] * inv[2][k];
+                    s += g * g;
+                }
+                f[a] += s * w;
+            }
+        } else {
+            double gu[3], p[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) gu[k] = (h[0] * inv[0][k] + h[1] * inv[1][k] + h[2] * inv[2][k]) * w;
+#pragma unroll
+            for (int d = 0; d < 3; d++) p[d] = inv[d][0] * gu[0] + inv[d][1] * gu[1] + inv[d][2] * gu[2];
+#pragma unroll
+            for (int a = 0; a < NN; a++) f[a] += t[3 * a] * p[0] + t[3 * a + 1] * p[1] + t[3 * a + 2] * p[2];
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < NN; a++) {
+        if constexpr (ATOMIC) unsafeAtomicAdd(&out[m[a] & kMask], f[a]);
+        else out[ve_base + slot * NN + a] = f[a];
+    }
+}
+
+// ---- hex20 isotropic elasticity -----------------------------------------
+// A wave takes EW = 64/QP element slots at a time; in phase 1 lane
+// l = e*QP + q holds element e at quadrature point q (QP: the point count
+// padded to 8, 32 or 64; padding points carry weight 0) and writes P (or inv
+// and w for DIAG) to an LDS slot; in phase 2 lane o = 3a+i < 60 sums the
+// output o of each element over the points.  xe: [slot][60], qp: [QP][61].
+// The grid is persistent: a block loads the dN table into LDS once and its
+// waves loop over element groups.  LDS strides avoid bank conflicts of the
+// 16-byte reads: dN rows [q][a][4] at kTabRow doubles (2*kTabRow = 4 * odd
+// dwords, so 8 rows hit 8 distinct bank groups), node values [e][a][x y z
+// u_x u_y u_z] at kElemRow doubles per element (likewise); the P slots
+// overlay the node values once phase 1 has read them.
+constexpr int kH20Slot = 12;   // S[4i+d] = P_di (DIAG: S[4k+d] = inv[d][k]), S[3] = w
+constexpr int kTabRow = 82;    // 80 used
+constexpr int kElemRow = 122;  // 120 used
+
+template <int QP> constexpr int h20_wave_doubles()
+{
+    return (64 / QP) * kElemRow > 64 * kH20Slot ? (64 / QP) * kElemRow : 64 * kH20Slot;
+}
+
+template <int QP, bool MASKED, bool ATOMIC, bool DIAG>
+__global__ __launch_bounds__(256) void free_hex20_kernel(const double* __restrict__ xe, const double* __restrict__ scale,
+                                                        const uint32_t* __restrict__ map, const double* __restrict__ u,
+                                                        double* __restrict__ out, uint64_t ve_base, uint32_t slot_begin,
+                                                        uint32_t n_slots, const double* __restrict__ qp, uint32_t nq,
+                                                        double c1, double c2, double c3,
+                                                        const uint32_t* __restrict__ skip)
+{
+    constexpr int EW = 64 / QP, WD = h20_wave_doubles<QP>();
+    __shared__ __align__(16) double tab[QP * kTabRow];
+    __shared__ double wq[QP];
+    __shared__ __align__(16) double work[4][WD];
+    if (skip && *skip) return;
+    for (uint32_t i = threadIdx.x; i < QP * 80u; i += blockDim.x) {
+        const uint32_t q = i / 80u, r = i - q * 80u, a = r >> 2, d = r & 3u;
+        tab[q * kTabRow + r] = d < 3 ? qp[q * 61u + 3u * a + d] : 0.0;
+    }
+    for (uint32_t q = threadIdx.x; q < QP; q += blockDim.x) wq[q] = qp[q * 61u + 60u];
+    __syncthreads();
+
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    double* W = work[wv];
+    const uint32_t n_groups = (n_slots + EW - 1) / EW;
+    // the lane's dN rows for phase 2 (8 points: kept in registers)
+    const uint32_t pa = lane < 60 ? lane / 3u : 19u, pi = lane < 60 ? lane - 3u * pa : 0u;
+    double dnr[QP == 8 ? 8 : 1][3];
+    if constexpr (QP == 8) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const double2 t01 = *reinterpret_cast<const double2*>(tab + q * kTabRow + 4 * pa);
+            dnr[q][0] = t01.x;
+            dnr[q][1] = t01.y;
+            dnr[q][2] = tab[q * kTabRow + 4 * pa + 2];
+        }
+    }
+
+    for (uint32_t grp = blockIdx.x * 4u + wv; grp < n_groups; grp += gridDim.x * 4u) {  // uniform per wave
+        const uint32_t first = grp * EW;
+        const uint32_t n_here = min((uint32_t)EW, n_slots - first);
+        const uint64_t slot0 = (uint64_t)slot_begin + first;
+
+        // cooperative load: element e's node a at W[e*kElemRow + 6a]: x y z, then u
+        for (uint32_t idx = lane; idx < EW * 60u; idx += 64u) {
+            const uint32_t e = idx / 60u, r = idx - e * 60u, a = r / 3u, k = r - 3u * a;
+            double xv = 0.0, uv = 0.0;
+            if (e < n_here) {
+                const uint64_t src = (slot0 + e) * 60u + r;
+                xv = xe[src];
+                if constexpr (!DIAG) {
+                    const uint32_t mm = map[src];
+                    uv = (MASKED && (mm & kFlag)) ? 0.0 : u[mm & kMask];
+                }
+            }
+            W[e * kElemRow + 6u * a + k] = xv;
+            W[e * kElemRow + 6u * a + 3u + k] = uv;
+        }
+        lds_sync_wave();
+
+        // phase 1
+        {
+            const uint32_t e = lane / QP, q = lane % QP;
+            const double* XU = W + e * kElemRow;
+            const double* T = tab + q * kTabRow;
+            double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll 4
+            for (int a = 0; a < 20; a++) {
+                const double2 t01 = *reinterpret_cast<const double2*>(T + 4 * a);
+                const double dn[3] = {t01.x, t01.y, T[4 * a + 2]};
+                const double2 p0 = *reinterpret_cast<const double2*>(XU + 6 * a);
+                const double2 p1 = *reinterpret_cast<const double2*>(XU + 6 * a + 2);
+                const double xs[3] = {p0.x, p0.y, p1.x};
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+#pragma unroll
+                    for (int d = 0; d < 3; d++) J[k][d] += xs[k] * dn[d];
+                if constexpr (!DIAG) {
+                    const double2 p2 = *reinterpret_cast<const double2*>(XU + 6 * a + 4);
+                    const double us[3] = {p1.y, p2.x, p2.y};
+#pragma unroll
+                    for (int k = 0; k < 3; k++)
+#pragma unroll
+                        for (int d = 0; d < 3; d++) H[k][d] += us[k] * dn[d];
+                }
+            }
+            double inv[3][3];
+            const double det = jac_inv_r(J, inv);
+            const double w = e < n_here ? det * wq[q] * scale[slot0 + e] : 0.0;
+            double S[kH20Slot];
+            if constexpr (DIAG) {
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+#pragma unroll
+                    for (int d = 0; d < 3; d++) S[4 * k + d] = inv[d][k];
+            } else {
+                double g[3][3];  // g[i][k] = du_i/dx_k
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int k = 0; k < 3; k++)
+                        g[i][k] = H[i][0] * inv[0][k] + H[i][1] * inv[1][k] + H[i][2] * inv[2][k];
+                const double exx = g[0][0] * w, eyy = g[1][1] * w, ezz = g[2][2] * w;
+                double sg[3][3];
+                sg[0][0] = c1 * exx + c2 * eyy + c2 * ezz;
+                sg[1][1] = c2 * exx + c1 * eyy + c2 * ezz;
+                sg[2][2] = c2 * exx + c2 * eyy + c1 * ezz;
+                sg[0][1] = sg[1][0] = (c3 * w) * (g[0][1] + g[1][0]);
+                sg[1][2] = sg[2][1] = (c3 * w) * (g[1][2] + g[2][1]);
+                sg[0][2] = sg[2][0] = (c3 * w) * (g[0][2] + g[2][0]);
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int d = 0; d < 3; d++)
+                        S[4 * i + d] = inv[d][0] * sg[i][0] + inv[d][1] * sg[i][1] + inv[d][2] * sg[i][2];
+            }
+            S[3] = w;
+            S[7] = S[11] = 0.0;
+            lds_sync_wave();  // every lane's reads of the node values precede the slot writes that overlay them
+            double* slotp = W + lane * kH20Slot;
+#pragma unroll
+            for (int j = 0; j < kH20Slot; j += 2) *reinterpret_cast<double2*>(slotp + j) = make_double2(S[j], S[j + 1]);
+        }
+        lds_sync_wave();
+
+        // phase 2: f[e] = sum_q dN_q(a) . P_q(:, i) of element e
+        double f[EW];
+#pragma unroll
+        for (int e = 0; e < EW; e++) f[e] = 0.0;
+        auto point = [&](uint32_t e, uint32_t q, double d0, double d1, double d2) {
+            const double* S = W + (e * QP + q) * kH20Slot;
+            if constexpr (DIAG) {
+                double g[3];
+#pragma unroll
+                for (int k = 0; k < 3; k++) g[k] = d0 * S[4 * k] + d1 * S[4 * k + 1] + d2 * S[4 * k + 2];
+                const double gi = pi == 0 ? g[0] : (pi == 1 ? g[1] : g[2]);
+                const double gg = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+                f[e] += (c1 * gi * gi + c3 * (gg - gi * gi)) * S[3];
+            } else {
+                const double2 p01 = *reinterpret_cast<const double2*>(S + 4 * pi);
+                const double p2 = S[4 * pi + 2];
+                f[e] += d0 * p01.x + d1 * p01.y + d2 * p2;
+            }
+        };
+        if constexpr (QP == 8) {
+#pragma unroll
+            for (uint32_t e = 0; e < EW; e++)
+#pragma unroll
+                for (uint32_t q = 0; q < 8; q++)
+                    if (q < nq) point(e, q, dnr[q][0], dnr[q][1], dnr[q][2]);
+        } else {
+            for (uint32_t q = 0; q < nq; q++) {
+                const double2 t01 = *reinterpret_cast<const double2*>(tab + q * kTabRow + 4 * pa);
+                const double t2 = tab[q * kTabRow + 4 * pa + 2];
+#pragma unroll
+                for (uint32_t e = 0; e < EW; e++) point(e, q, t01.x, t01.y, t2);
+            }
+        }
+        if (lane < 60) {
+#pragma unroll
+            for (uint32_t e = 0; e < EW; e++) {
+                if (e < n_here) {
+                    const uint64_t slot = slot0 + e;
+                    if constexpr (ATOMIC) unsafeAtomicAdd(&out[map[slot * 60u + lane] & kMask], f[e]);
+                    else out[ve_base + slot * 60u + lane] = f[e];
+                }
+            }
+        }
+        lds_sync_wave();  // phase-2 reads of the slots precede the next group's node writes
+    }
+}
+
+template <int QP>
+void launch_hex20(bool masked, bool atomic, bool diag, const FreeArgs& a, hipStream_t s)
+{
+    constexpr uint32_t per_block = 4u * (64u / QP);
+    static int resident = 0;  // blocks per device: CUs x blocks per CU (the LDS bounds it)
+    if (resident == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        AMX_HIP(hipGetDevice(&dev));
+        AMX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        AMX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, free_hex20_kernel<QP, false, false, false>,
+                                                             256, 0));
+        resident = std::max(1, cus * std::max(per_cu, 1));
+    }
+    const uint32_t need = (a.n_slots + per_block - 1) / per_block;
+    const dim3 grid(std::min<uint32_t>(need, (uint32_t)resident)), block(256);
+#define AMX_H20(M, A, D)                                                                                           \
+    free_hex20_kernel<QP, M, A, D><<<grid, block, 0, s>>>(a.xe, a.scale, a.map, a.u, a.out, a.ve_base,            \
+                                                          a.slot_begin, a.n_slots, a.qp, a.nq, a.c1, a.c2, a.c3,  \
+                                                          a.skip)
+    if (diag) AMX_H20(false, false, true);
+    else if (masked) { if (atomic) AMX_H20(true, true, false); else AMX_H20(true, false, false); }
+    else { if (atomic) AMX_H20(false, true, false); else AMX_H20(false, false, false); }
+#undef AMX_H20
+}
+
+template <int NN>
+void launch_laplace(bool masked, bool atomic, bool diag, const FreeArgs& a, hipStream_t s)
+{
+    const dim3 grid((a.n_slots + 255) / 256), block(256);
+#define AMX_LAP(M, A, D)                                                                                           \
+    free_laplace_kernel<NN, M, A, D><<<grid, block, 0, s>>>(a.xe, a.scale, a.map, a.u, a.out, a.ve_base,          \
+                                                            a.slot_begin, a.n_slots, a.stride, a.qp, a.nq, a.skip)
+    if (diag) AMX_LAP(false, false, true);
+    else if (masked) { if (atomic) AMX_LAP(true, true, false); else AMX_LAP(true, false, false); }
+    else { if (atomic) AMX_LAP(false, true, false); else AMX_LAP(false, false, false); }
+#undef AMX_LAP
+}
+
+}  // namespace
+
+uint32_t free_padded_points(int kind, uint32_t nq)
+{
+    if (kind != AMX_ELEM_HEX20_ELASTIC) return nq;
+    return nq <= 8 ? 8 : (nq <= 32 ? 32 : 64);
+}
+
+void launch_free(int kind, bool masked, bool atomic, bool diag, const FreeArgs& a, hipStream_t s)
+{
+    if (a.n_slots == 0) return;
+    switch (kind) {
+    case AMX_ELEM_HEX8_LAPLACE: launch_laplace<8>(masked, atomic, diag, a, s); break;
+    case AMX_ELEM_TET10_LAPLACE: launch_laplace<10>(masked, atomic, diag, a, s); break;
+    case AMX_ELEM_HEX20_ELASTIC: {
+        AMX_REQUIRE(a.nq <= 64, AMX_NOT_IMPLEMENTED, "hex20: at most 64 quadrature points");
+        const uint32_t qp = free_padded_points(kind, a.nq);
+        if (qp == 8) launch_hex20<8>(masked, atomic, diag, a, s);
+        else if (qp == 32) launch_hex20<32>(masked, atomic, diag, a, s);
+        else launch_hex20<64>(masked, atomic, diag, a, s);
+        break;
+    }
+    default: fail(AMX_INVALID_ARGUMENT, "unknown element kind " + std::to_string(kind));
+    }
+    AMX_HIP(hipGetLastError());
+}
+
+}  // namespace amx

@@ -88,6 +88,10 @@ struct DevBuf {
     template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 
+// quadrature table of the FREE element kernels (amx_fem.cpp): nq points of
+// [dN/dxi per node (3 each), weight], padded with zero-weight rows to rows_min
+std::vector<double> free_quadrature(int kind, uint32_t ngt, uint32_t& nq, uint32_t rows_min);
+
 }  // namespace amx
 
 // Maps handle (host side), see amx_maps.cpp.

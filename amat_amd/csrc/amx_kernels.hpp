@@ -92,6 +92,24 @@ void launch_ebe_batched(uint32_t n, KType kt, bool vt_double, const EbeJob* jobs
 void launch_gather_batched(bool vt_double, const GatherJob* jobs, const uint32_t* prefix, uint32_t n_jobs,
                            uint32_t total_blocks, hipStream_t s);
 
+// ---- method 2 (FREE): element kernels from coordinates (amx_free.hip) ---
+struct FreeArgs {
+    const double* xe;        // Laplace kinds: [3*nodes][stride]; hex20: [slot][60]
+    const double* scale;     // [slot] stiffness factor (0 on padding slots)
+    const uint32_t* map;     // as EbeArgs::map
+    const double* u;
+    double* out;             // ve (deterministic) or v (atomic)
+    uint64_t ve_base;
+    uint32_t slot_begin, n_slots, stride;
+    const double* qp;        // [points][3*nodes+1]: dN/dxi per node, weight
+    uint32_t nq;             // quadrature points (hex20: table padded to free_padded_points rows)
+    double c1, c2, c3;       // isotropic constitutive entries (hex20)
+    const uint32_t* skip = nullptr;
+};
+// kind: amx_element_kind.  diag: write diag(s_e K_e) into ve rows instead.
+void launch_free(int kind, bool masked, bool atomic, bool diag, const FreeArgs& a, hipStream_t s);
+uint32_t free_padded_points(int kind, uint32_t nq);
+
 // constrained rows after an atomic scatter or a halo accumulation
 void launch_bc_fixup(bool vt_double, int bc, const uint32_t* cdofs, uint32_t nc, const double* coef,
                      const void* u, void* v, hipStream_t s);

@@ -707,6 +707,7 @@ int amx_mrhs_create(amx_op* const* ops, int n_ops, int precision, amx_mrhs** out
         for (int i = 0; i < n_ops; i++) {
             amx::Op& op = ops[i]->op;
             AMX_REQUIRE(op.finalized, AMX_INVALID_STATE, "operator not finalized");
+            AMX_REQUIRE(!op.is_free(), AMX_NOT_IMPLEMENTED, "multi-RHS products need stored element matrices");
             AMX_REQUIRE(op.device == m->device, AMX_INVALID_ARGUMENT, "operators on different devices");
             AMX_REQUIRE(op.groups.size() == 1, AMX_NOT_IMPLEMENTED, "one element type per operator");
             const amx::Group& g = op.groups[0];

@@ -44,6 +44,51 @@ KType ktype_of(int precision)
     fail(AMX_INVALID_ARGUMENT, "unknown precision " + std::to_string(precision));
 }
 
+// FREE: coordinates and factors in slot order (hex20 [slot][60], the Laplace
+// kinds [coordinate][slot]); padding slots repeat the group's first element
+// with factor 0, so that they add exact zeros.
+void upload_free(Op& op, Group& g, hipStream_t s)
+{
+    const uint32_t nx = 3 * free_nodes(op.elem_kind);
+    const bool aos = op.elem_kind == AMX_ELEM_HEX20_ELASTIC;
+    std::vector<double> hx((size_t)g.n_slots * nx), hs(g.n_slots, 0.0);
+    for (uint32_t sl = 0; sl < g.n_slots; sl++) {
+        const uint32_t i = g.slot_elem[sl];
+        const uint32_t e = g.elems[i == kPadSlot ? 0 : i];
+        const double* src = op.elem_xe.data() + (size_t)e * nx;
+        if (i != kPadSlot) hs[sl] = op.elem_scale[e];
+        for (uint32_t c = 0; c < nx; c++) {
+            if (aos) hx[(size_t)sl * nx + c] = src[c];
+            else hx[(size_t)c * g.n_slots + sl] = src[c];
+        }
+    }
+    upload(g.xe, hx, s, op.device_bytes);
+    upload(g.fscale, hs, s, op.device_bytes);
+    AMX_HIP(hipStreamSynchronize(s));  // host vectors leave scope
+}
+
+FreeArgs free_args(const Op& op, const Group& g, const void* u, void* out, uint32_t slot_begin, uint32_t n_slots,
+                   const uint32_t* skip)
+{
+    FreeArgs a;
+    a.xe = g.xe.as<double>();
+    a.scale = g.fscale.as<double>();
+    a.map = g.map.as<uint32_t>();
+    a.u = static_cast<const double*>(u);
+    a.out = static_cast<double*>(out);
+    a.ve_base = g.ve_base;
+    a.slot_begin = slot_begin;
+    a.n_slots = n_slots;
+    a.stride = g.n_slots;
+    a.qp = op.qp.as<double>();
+    a.nq = op.elem_nq;
+    a.c1 = op.elem_c[0];
+    a.c2 = op.elem_c[1];
+    a.c3 = op.elem_c[2];
+    a.skip = skip;
+    return a;
+}
+
 // Build slots, device K, maps, transposed lists and BC tables.
 void finalize_op(Op& op)
 {
@@ -51,8 +96,21 @@ void finalize_op(Op& op)
     hipStream_t s = op.stream;
     // free previous device state (finalize may be called again after updates)
     op.device_bytes = 0;
-    for (auto& g : op.groups)
+    for (auto& g : op.groups) {
         for (auto& k : g.K) k.release();
+        g.kdiag16.release();
+        g.xe.release();
+        g.fscale.release();
+    }
+    const bool free_m = op.is_free();
+    if (free_m) {
+        AMX_REQUIRE(op.elem_kind != 0, AMX_INVALID_STATE, "method 2 (FREE) needs set_element_function before finalize");
+        uint32_t nq = 0;
+        const std::vector<double> tab =
+            free_quadrature(op.elem_kind, op.elem_ngt, nq, free_padded_points(op.elem_kind, op.elem_nq));
+        upload(op.qp, tab, s, op.device_bytes);
+        AMX_HIP(hipStreamSynchronize(s));
+    }
 
     // flagged local ids: zeroed on gather by the identity BC form -- owned
     // constrained DoFs (MatMult_mf :3916-3922) and the element DoFs the BC map
@@ -100,6 +158,10 @@ void finalize_op(Op& op)
             }
         }
         upload(g.map, hmap, s, op.device_bytes);
+        if (free_m) {
+            upload_free(op, g, s);
+            continue;
+        }
 
         // element matrices: upload the source once, relayout on the device
         DevBuf src, src_scale, dslot;
@@ -191,13 +253,35 @@ void finalize_op(Op& op)
     upload(op.diag_bc_ids, dbc, s, op.device_bytes);
     op.n_diag_bc = (uint32_t)dbc.size();
 
+    // ---- element diagonals into ve (FREE: computed from the coordinates,
+    // mat_get_diagonal_ghosted_free)
+    for (auto& g : op.groups) {
+        if (free_m)
+            launch_free(op.elem_kind, false, false, true, free_args(op, g, nullptr, op.ve.p, 0, g.n_slots, nullptr), s);
+        else
+            launch_extract_diag(g.K[0].as<double>(), g.L[0], g.n_slots, op.ve.as<double>() + g.ve_base, s);
+    }
+
     // ---- trace in element order (finalize_end :748-785)
     double tr = 0.0;
-    for (uint32_t e = 0; e < op.n_elem; e++) {
-        const Group& g = op.groups[op.elem_group[e]];
-        const uint32_t n = g.n, i = op.elem_index[e];
-        for (uint32_t r = 0; r < n; r++)
-            tr += op.scaled ? op.scale[e] * op.kref[(size_t)r * n + r] : g.stage[(size_t)i * n * n + (size_t)r * n + r];
+    if (free_m) {
+        std::vector<double> hve(ve_rows);
+        if (ve_rows)
+            AMX_HIP(hipMemcpyAsync(hve.data(), op.ve.p, ve_rows * sizeof(double), hipMemcpyDeviceToHost, s));
+        AMX_HIP(hipStreamSynchronize(s));
+        for (uint32_t e = 0; e < op.n_elem; e++) {
+            const Group& g = op.groups[op.elem_group[e]];
+            const uint64_t base = g.ve_base + (uint64_t)g.elem_slot[op.elem_index[e]] * g.n;
+            for (uint32_t r = 0; r < g.n; r++) tr += hve[base + r];
+        }
+    } else {
+        for (uint32_t e = 0; e < op.n_elem; e++) {
+            const Group& g = op.groups[op.elem_group[e]];
+            const uint32_t n = g.n, i = op.elem_index[e];
+            for (uint32_t r = 0; r < n; r++)
+                tr += op.scaled ? op.scale[e] * op.kref[(size_t)r * n + r]
+                                : g.stage[(size_t)i * n * n + (size_t)r * n + r];
+        }
     }
     op.trace = tr;
 
@@ -205,8 +289,6 @@ void finalize_op(Op& op)
     // element order as the gather, and the penalty coefficients p - diag
     op.diag.alloc(std::max<uint32_t>(op.n_total, 1) * sizeof(double));
     op.device_bytes += op.diag.bytes;
-    for (auto& g : op.groups)
-        launch_extract_diag(g.K[0].as<double>(), g.L[0], g.n_slots, op.ve.as<double>() + g.ve_base, s);
     launch_gather(true, 0, op.off.as<uint32_t>(), op.pos.as<uint32_t>(), op.ve.p, nullptr, nullptr, nullptr,
                   op.diag.p, op.n_total, s);
     op.coef64.alloc(std::max<uint32_t>(op.n_total, 1) * sizeof(double));
@@ -280,6 +362,17 @@ void ebe_launch(Op& op, KType kt, bool vt_double, bool masked, int phase, const 
 {
     if (!ve) ve = op.ve.p;
     const bool atomic = op.scatter == AMX_SCATTER_ATOMIC;
+    if (op.is_free()) {
+        AMX_REQUIRE(kt == KType::F64 && vt_double, AMX_NOT_IMPLEMENTED, "method 2 (FREE) runs in FP64 only");
+        for (auto& g : op.groups) {
+            uint32_t t0 = 0, nt = g.n_tiles;
+            if (phase == 0) nt = g.tiles_phase0;
+            else if (phase == 1) { t0 = g.tiles_phase0; nt = g.n_tiles - t0; }
+            launch_free(op.elem_kind, masked, atomic, false,
+                        free_args(op, g, u, atomic ? v : ve, t0 * g.G, nt * g.G, skip), s);
+        }
+        return;
+    }
     for (auto& g : op.groups) {
         const int k = (int)kt;
         AMX_REQUIRE(g.K[k].p != nullptr, AMX_INVALID_STATE, "requested precision was not built at finalize");
@@ -408,10 +501,11 @@ int amx_op_set_matfree_type(amx_op* h, int type)
 {
     return guarded(__func__, [&] {
         AMX_REQUIRE(h, AMX_INVALID_ARGUMENT, "op is NULL");
-        if (type == AMX_MATFREE_QUASI_FREE || type == AMX_MATFREE_FREE)
-            fail(AMX_NOT_IMPLEMENTED, "methods 1 and 2 are the reference's CPU loops; this library runs 3/4/5");
-        AMX_REQUIRE(type >= AMX_MATFREE_GPU_PURE && type <= AMX_MATFREE_GPU_OVER_GPU, AMX_UNKNOWN_MAT_TYPE,
+        if (type == AMX_MATFREE_QUASI_FREE)
+            fail(AMX_NOT_IMPLEMENTED, "method 1 is the reference's CPU loop; this library runs 2 to 5");
+        AMX_REQUIRE(type >= AMX_MATFREE_FREE && type <= AMX_MATFREE_GPU_OVER_GPU, AMX_UNKNOWN_MAT_TYPE,
                     "unknown matrix-free type " + std::to_string(type));
+        if (type != h->op.matfree_type && (type == AMX_MATFREE_FREE || h->op.is_free())) h->op.finalized = false;
         h->op.matfree_type = type;
     });
 }
@@ -464,6 +558,39 @@ int amx_op_set_element_matrices_scaled(amx_op* h, const double* k_ref, int layou
         op.groups[0].stage.clear();
         op.groups[0].stage.shrink_to_fit();
         std::fill(op.elem_bdim.begin(), op.elem_bdim.end(), 1u);
+        op.finalized = false;
+    });
+}
+
+int amx_op_set_element_function(amx_op* h, int kind, const double* xe, const double* scale, double E, double nu,
+                                uint32_t ngt)
+{
+    return guarded(__func__, [&] {
+        AMX_REQUIRE(h && xe, AMX_INVALID_ARGUMENT, "NULL argument");
+        Op& op = h->op;
+        AMX_REQUIRE(kind == AMX_ELEM_HEX8_LAPLACE || kind == AMX_ELEM_HEX20_ELASTIC || kind == AMX_ELEM_TET10_LAPLACE,
+                    AMX_UNKNOWN_ELEMENT_TYPE, "unknown element kind " + std::to_string(kind));
+        const uint32_t nodes = amx::free_nodes(kind), ndof = kind == AMX_ELEM_HEX20_ELASTIC ? 60u : nodes;
+        for (uint32_t e = 0; e < op.n_elem; e++)
+            AMX_REQUIRE(op.dpe[e] == ndof, AMX_INVALID_ARGUMENT,
+                        "element " + std::to_string(e) + " has " + std::to_string(op.dpe[e]) + " DoFs; the kind has " +
+                            std::to_string(ndof));
+        uint32_t nq = 0;
+        std::vector<double> tab = amx::free_quadrature(kind, ngt, nq, 0);
+        if (kind == AMX_ELEM_HEX20_ELASTIC) {
+            AMX_REQUIRE(nq <= 64, AMX_NOT_IMPLEMENTED, "hex20: at most 64 quadrature points (ngt <= 4)");
+            AMX_REQUIRE(E > 0.0 && nu > -1.0 && nu < 0.5, AMX_INVALID_ARGUMENT, "need E > 0 and -1 < nu < 0.5");
+            op.elem_c[0] = E * (1.0 - nu) / (1.0 + nu) / (1.0 - 2.0 * nu);
+            op.elem_c[1] = E * nu / (1.0 + nu) / (1.0 - 2.0 * nu);
+            op.elem_c[2] = 0.5 * E / (1.0 + nu);
+        }
+        op.elem_kind = kind;
+        op.elem_ngt = ngt;
+        op.elem_nq = nq;
+        op.elem_xe.assign(xe, xe + (size_t)op.n_elem * 3 * nodes);
+        if (scale) op.elem_scale.assign(scale, scale + op.n_elem);
+        else op.elem_scale.assign(op.n_elem, 1.0);
+        op.matfree_type = AMX_MATFREE_FREE;
         op.finalized = false;
     });
 }
@@ -539,6 +666,11 @@ int amx_op_get_info(const amx_op* h, amx_op_info* info)
         info->device_bytes = op.device_bytes;
         for (const auto& g : op.groups) {
             const uint64_t ne = g.elems.size(), nn = (uint64_t)g.n * g.n;
+            if (op.is_free()) {  // coordinates, factor and map read; no element matrices
+                info->alg_bytes_fp64 += ne * (24ull * amx::free_nodes(op.elem_kind) + 8 + 4ull * g.n);
+                info->flops += ne * amx::free_flops_per_elem(op.elem_kind, op.elem_nq);
+                continue;
+            }
             info->k_bytes_fp64 += ne * nn * 8;
             info->k_bytes_fp32 += ne * nn * 4;
             info->k_bytes_bf16 += ne * nn * 2;

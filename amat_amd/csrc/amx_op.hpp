@@ -28,6 +28,7 @@ struct Group {
     DevBuf K[3];                      // tiled storage per KType (F64 always)
     DevBuf map;                       // [slot][r] local ids | identity-BC flag
     DevBuf kdiag16;                   // BF16 storage: FP32 corrected diagonal [slot][r]
+    DevBuf xe, fscale;                // FREE: coordinates and factors per slot
 };
 
 struct Op {
@@ -68,6 +69,12 @@ struct Op {
     std::vector<double> kref;            // row-major, scaled mode
     std::vector<double> scale;           // per element, scaled mode
     std::vector<Group> groups;
+    // method 2 (FREE): element kind, coordinates [e][3*nodes], factors
+    int elem_kind = 0;
+    uint32_t elem_ngt = 0, elem_nq = 0;
+    double elem_c[3] = {0.0, 0.0, 0.0};
+    std::vector<double> elem_xe, elem_scale;
+    DevBuf qp;
 
     // device tables
     uint64_t ve_rows = 0;
@@ -93,7 +100,22 @@ struct Op {
     // the caller's stream; NULL is the HIP default stream (not the handle's)
     hipStream_t pick(void* s) const { return static_cast<hipStream_t>(s); }
     bool has(int prec_bit) const { return (prec_mask & prec_bit) != 0; }
+    bool is_free() const { return matfree_type == AMX_MATFREE_FREE; }
 };
+
+// nodes of a FREE element kind (amx_element_kind)
+inline uint32_t free_nodes(int kind)
+{
+    return kind == AMX_ELEM_HEX8_LAPLACE ? 8u : (kind == AMX_ELEM_HEX20_ELASTIC ? 20u : 10u);
+}
+// algorithmic FP64 flops of one FREE element application (amx_free.hip):
+// Laplace per point 30 n + 85 (Jacobian 18 n, h 6 n, output 6 n, inverse and
+// transforms 85); hex20 per point 1260 (Jacobian 360, H 360, output 360,
+// inverse, strain, stress and P 180)
+inline uint64_t free_flops_per_elem(int kind, uint32_t nq)
+{
+    return (uint64_t)nq * (kind == AMX_ELEM_HEX20_ELASTIC ? 1260u : 30u * free_nodes(kind) + 85u);
+}
 
 // RAII: make the handle's device current, restore the previous one on exit.
 struct DeviceGuard {

@@ -374,6 +374,7 @@ int amx_server_submit(amx_server* S, amx_op* h, int method, const double* b_host
                     "inner_rtol must be in (0, 1)");
         Op& op = h->op;
         AMX_REQUIRE(op.finalized, AMX_INVALID_STATE, "operator not finalized");
+        AMX_REQUIRE(!op.is_free(), AMX_NOT_IMPLEMENTED, "served solves need stored element matrices");
         AMX_REQUIRE(op.size == 1, AMX_NOT_IMPLEMENTED, "served solves are one-rank");
         AMX_REQUIRE(op.device == S->device, AMX_INVALID_ARGUMENT, "operator on another device");
         AMX_REQUIRE(op.scatter == AMX_SCATTER_DETERMINISTIC, AMX_NOT_IMPLEMENTED,

@@ -35,6 +35,8 @@ class Problem:
     constrained: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, np.uint64))
     prescribed: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
     rhs: np.ndarray | None = None
+    # method 2 (FREE) description of the same K_e: kind, ngt, E, nu
+    free: dict | None = None
 
     @property
     def n_elem(self) -> int:
@@ -74,6 +76,27 @@ class Problem:
         op.finalize()
         return op
 
+    def element_coords(self) -> np.ndarray:
+        """(E, nodes, 3) node coordinates in element DoF-node order."""
+        return np.ascontiguousarray(self.mesh.coords[self.mesh.elem_nodes])
+
+    def build_free_operator(self, aMatFree, Maps, bc_meth=0, scatter=0, device=0, bcop=None, penalty=None):
+        """The same operator by method 2 (FREE): K_e computed in-kernel from the
+        coordinates at every application."""
+        f = self.free
+        maps = self.build_maps(Maps)
+        op = aMatFree(maps, bc_meth, device)
+        op.set_matfree_type(2)
+        op.set_element_function(f["kind"], self.element_coords(), self.scale, f.get("E", 1.0), f.get("nu", 0.3),
+                                f["ngt"])
+        op.set_options(1, scatter)
+        if bcop is not None:
+            op.set_bc_operator(bcop)
+        if penalty is not None:
+            op.set_penalty(penalty)
+        op.finalize()
+        return op
+
     def random_u(self, seed: int = SEED_U) -> np.ndarray:
         return meshmod.uniform(seed, 0, self.n_dofs, -1.0, 1.0)
 
@@ -93,7 +116,7 @@ def poisson_hex8(n: int, ny: int | None = None, nz: int | None = None, L: float 
     kref = fem.ke_hex8(xe, 2)
     cons = meshmod.boundary_nodes(m).astype(np.uint64)
     return Problem("poisson_hex8", m, 8, m.dof_map(), k_ref=kref, scale=np.ones(m.n_elem), constrained=cons,
-                   prescribed=np.zeros(len(cons)))
+                   prescribed=np.zeros(len(cons)), free={"kind": 1, "ngt": 2})
 
 
 def elasticity_hex20(n: int, ny: int | None = None, nz: int | None = None, E: float = 1.0, nu: float = 0.3,
@@ -107,7 +130,7 @@ def elasticity_hex20(n: int, ny: int | None = None, nz: int | None = None, E: fl
     z0 = meshmod.boundary_nodes(m, faces=("z0",)).astype(np.uint64)
     cons = np.sort(np.concatenate([z0 * np.uint64(3) + np.uint64(c) for c in range(3)]))
     return Problem("elasticity_hex20", m, 60, m.dof_map(), k_ref=kref, scale=scale, constrained=cons,
-                   prescribed=np.zeros(len(cons)))
+                   prescribed=np.zeros(len(cons)), free={"kind": 2, "ngt": ngt, "E": E, "nu": nu})
 
 
 def poisson_tet10(n: int, seed: int = 3, jitter: float = 0.2, with_rhs: bool = True) -> Problem:
@@ -122,4 +145,4 @@ def poisson_tet10(n: int, seed: int = 3, jitter: float = 0.2, with_rhs: bool = T
         fe = fem.fe_tet10_batch(coords, f[m.elem_nodes], 2)
         rhs = np.bincount(m.elem_nodes.reshape(-1).astype(np.int64), weights=fe.reshape(-1), minlength=m.n_nodes)
     return Problem("poisson_tet10", m, 10, m.dof_map(), K=K, constrained=cons, prescribed=np.zeros(len(cons)),
-                   rhs=rhs)
+                   rhs=rhs, free={"kind": 3, "ngt": 2})

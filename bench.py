@@ -147,6 +147,80 @@ def precision_mode(args) -> dict:
     return res
 
 
+FP64_PEAK = 78.6e12  # MI355X FP64 (vector and matrix), flop/s
+
+
+def free_mode(args) -> dict:
+    """Method 2 (FREE) on the headline workload: hex20 elasticity n^3, the same
+    operator with K_e computed in-kernel from the node coordinates at every
+    application (no element matrices in HBM), against the stored-matrix FP64
+    operator; Gauss rules 2^3 (the reference example's NGT) and 3^3."""
+    import torch
+    from amat_amd import BCOP, PREC, Maps, aMatFree, problems
+    res = {"metric": "method 2 (FREE) FP64 matvec vs stored K_e", "n": args.n}
+    s = torch.cuda.current_stream()
+    for ngt in (3, 2):
+        t0 = time.time()
+        p = problems.elasticity_hex20(args.n, ngt=ngt)
+        st = p.build_operator(aMatFree, Maps, precision=PREC.FP64, bcop=BCOP.PENALTY_DIAG, penalty=1e8)
+        fr = p.build_free_operator(aMatFree, Maps, bcop=BCOP.PENALTY_DIAG, penalty=1e8)
+        log(f"[free] hex20 {args.n}^3 ngt {ngt}: {p.n_elem} elements, {p.n_dofs} DoFs, operators built "
+            f"({time.time() - t0:.1f}s)")
+        u = torch.from_numpy(p.random_u()).cuda()
+        v0, v1 = torch.empty_like(u), torch.empty_like(u)
+        st.matvec_dev(v0, u, s)   # K u without the BC rows, which the penalty would dominate
+        fr.matvec_dev(v1, u, s)
+        torch.cuda.synchronize()
+        rel = float(torch.linalg.norm(v1 - v0) / torch.linalg.norm(v0))
+        N, E = p.n_dofs, p.n_elem
+        for name, op in (("stored", st), ("free", fr)):
+            ms = time_matvec(op.matmult_dev, v0, u, args.steps, args.warmup, torch)
+            ms_e, ms_g, _ = op.time_kernels_dev(args.steps, v0, u, s)
+            inf = op.info()
+            r = {"ms": ms, "GDoF/s": N / ms / 1e6, "element_kernel_ms": ms_e, "gather_ms": ms_g,
+                 "device_bytes": inf.device_bytes}
+            if name == "free":
+                r["flops_per_matvec"] = inf.flops
+                r["element_kernel_TFLOP/s"] = inf.flops / (ms_e * 1e-3) / 1e12
+                r["fp64_peak_frac"] = inf.flops / (ms_e * 1e-3) / FP64_PEAK
+                r["rel_l2_vs_stored"] = rel
+            else:
+                elem_bytes = E * (60 * 60 * 8 + 4 * 60) + N * 8
+                r["element_kernel_hbm_frac"] = elem_bytes / (ms_e * 1e-3) / HBM_PEAK
+            res[f"ngt{ngt}_{name}"] = r
+            log(f"[free] ngt {ngt} {name}: {ms:.4f} ms/matvec ({N / ms / 1e6:.2f} GDoF/s), element kernel "
+                f"{ms_e:.4f} ms, gather {ms_g:.4f} ms, device {inf.device_bytes / 1e9:.2f} GB"
+                + (f", {inf.flops / (ms_e * 1e-3) / 1e12:.2f} TFLOP/s, rel L2 vs stored {rel:.2e}"
+                   if name == "free" else ""))
+        res[f"ngt{ngt}_speedup"] = res[f"ngt{ngt}_stored"]["ms"] / res[f"ngt{ngt}_free"]["ms"]
+        del st, fr, u, v0, v1
+        torch.cuda.empty_cache()
+    # hex8 Poisson (2x2x2 Gauss), one lane per element
+    nh = 2 * args.n
+    p = problems.poisson_hex8(nh)
+    st = p.build_operator(aMatFree, Maps, bcop=BCOP.IDENTITY)
+    fr = p.build_free_operator(aMatFree, Maps, bcop=BCOP.IDENTITY)
+    u = torch.from_numpy(p.random_u()).cuda()
+    v0, v1 = torch.empty_like(u), torch.empty_like(u)
+    st.matvec_dev(v0, u, s)
+    fr.matvec_dev(v1, u, s)
+    torch.cuda.synchronize()
+    rel = float(torch.linalg.norm(v1 - v0) / torch.linalg.norm(v0))
+    for name, op in (("stored", st), ("free", fr)):
+        ms = time_matvec(op.matmult_dev, v0, u, args.steps, args.warmup, torch)
+        ms_e, ms_g, _ = op.time_kernels_dev(args.steps, v0, u, s)
+        inf = op.info()
+        res[f"hex8_{nh}_{name}"] = {"ms": ms, "GDoF/s": p.n_dofs / ms / 1e6, "element_kernel_ms": ms_e,
+                                    "gather_ms": ms_g, "device_bytes": inf.device_bytes,
+                                    "alg_bytes_per_matvec": inf.alg_bytes_fp64,
+                                    "hbm_frac": inf.alg_bytes_fp64 / (ms * 1e-3) / HBM_PEAK}
+        if name == "free":
+            res[f"hex8_{nh}_free"]["rel_l2_vs_stored"] = rel
+        log(f"[free] hex8 {nh}^3 {name}: {ms:.4f} ms/matvec ({p.n_dofs / ms / 1e6:.2f} GDoF/s), element kernel "
+            f"{ms_e:.4f} ms, gather {ms_g:.4f} ms")
+    return res
+
+
 MFMA_PEAK = {"fp32": 157.3e12, "bf16": 2.5e15}  # MI355X dense MFMA, flop/s
 
 
@@ -289,7 +363,7 @@ def serve_mode(args) -> dict:
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--mode", choices=["matvec", "precision", "batch", "serve"], default="matvec")
+    ap.add_argument("--mode", choices=["matvec", "precision", "batch", "serve", "free"], default="matvec")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rates", default="50,200,800")
     ap.add_argument("--serve-methods", default="fp64,bf16,fp32")
@@ -331,6 +405,9 @@ def main() -> int:
         return 0
     if args.mode == "serve":
         print(json.dumps(serve_mode(args)), flush=True)
+        return 0
+    if args.mode == "free":
+        print(json.dumps(free_mode(args)), flush=True)
         return 0
     t0 = time.time()
     p = problems.elasticity_hex20(args.n)

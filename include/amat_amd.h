@@ -61,8 +61,9 @@ enum amx_status {
 /* par::BC_METH, include/enums.hpp:20-22 */
 enum amx_bc_meth { AMX_BC_IMATRIX = 0, AMX_BC_PENALTY = 1 };
 
-/* par::MATFREE_TYPE, include/enums.hpp:56.  This library runs 3/4/5 on the
- * GPU; 1 and 2 are the reference's CPU loops and return AMX_NOT_IMPLEMENTED. */
+/* par::MATFREE_TYPE, include/enums.hpp:56.  This library runs 2 (FREE, with
+ * the built-in element kinds of amx_op_set_element_function) and 3/4/5 on the
+ * GPU; 1 is the reference's CPU loop and returns AMX_NOT_IMPLEMENTED. */
 enum amx_matfree_type {
     AMX_MATFREE_UNDEFINED = 0,
     AMX_MATFREE_QUASI_FREE = 1,
@@ -70,6 +71,15 @@ enum amx_matfree_type {
     AMX_MATFREE_GPU_PURE = 3,
     AMX_MATFREE_GPU_OVER_CPU = 4,
     AMX_MATFREE_GPU_OVER_GPU = 5
+};
+
+/* element kinds computed in-kernel by method 2 (FREE): the element matrices
+ * of the examples, ke_hex8 / ke_hex20_iso / ke_tet10
+ * (examples/src/ke_matrix.cpp:436-526, 1573-1740, 329-433) */
+enum amx_element_kind {
+    AMX_ELEM_HEX8_LAPLACE = 1,   /* 8 nodes, 1 DoF per node */
+    AMX_ELEM_HEX20_ELASTIC = 2,  /* 20 nodes, 3 DoFs per node (DoF 3a+i), isotropic */
+    AMX_ELEM_TET10_LAPLACE = 3   /* 10 nodes, 1 DoF per node */
 };
 
 /* storage precision of the element matrices (bit mask, amx_op_set_options) */
@@ -198,6 +208,19 @@ int amx_op_destroy(amx_op* op);
 
 /* aMat::set_matfree_type (include/aMat.hpp:184) */
 int amx_op_set_matfree_type(amx_op* op, int matfree_type);
+/* aMat::set_element_matrix_function (include/aMat.hpp:114-116) for method 2
+ * (FREE, include/aMatFree.hpp:2240-2744): instead of a callback that fills
+ * K_e from the element's coordinates (examples/src/example07a.cpp:54-70),
+ * the kernel computes s_e K_e u_e from the coordinates at every matvec.
+ * xe: n_elems x (3 * nodes) node coordinates, node-major in the element's DoF
+ * order (x0,y0,z0,x1,...); scale: per-element factor s_e (NULL: 1); E, nu:
+ * isotropic material (hex20); ngt: Gauss points per direction, as the
+ * examples' NGT (hex20: ngt^3 <= 64).  Every element must have the kind's DoF
+ * count.  Sets the matfree type to FREE; finalize() builds no element
+ * matrices.  FP64 only: the mixed-precision, multi-RHS and server entry
+ * points reject a FREE operator. */
+int amx_op_set_element_function(amx_op* op, int kind, const double* xe, const double* scale,
+                                double E, double nu, uint32_t ngt);
 /* aMatFree::set_num_streams (include/aMatFree.hpp:230-237): number of HIP
  * streams used by the two-phase (independent/dependent) launch; >= 1 */
 int amx_op_set_num_streams(amx_op* op, uint32_t n_streams);
