@@ -178,40 +178,68 @@ __global__ __launch_bounds__(256) void free_hex20_kernel(const double* __restric
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     double* W = work[wv];
     const uint32_t n_groups = (n_slots + EW - 1) / EW;
-    // the lane's dN rows for phase 2 (8 points: kept in registers)
     const uint32_t pa = lane < 60 ? lane / 3u : 19u, pi = lane < 60 ? lane - 3u * pa : 0u;
-    double dnr[QP == 8 ? 8 : 1][3];
-    if constexpr (QP == 8) {
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const double2 t01 = *reinterpret_cast<const double2*>(tab + q * kTabRow + 4 * pa);
-            dnr[q][0] = t01.x;
-            dnr[q][1] = t01.y;
-            dnr[q][2] = tab[q * kTabRow + 4 * pa + 2];
-        }
-    }
 
-    for (uint32_t grp = blockIdx.x * 4u + wv; grp < n_groups; grp += gridDim.x * 4u) {  // uniform per wave
+    // the next group's coordinates, map entries, element values and factor are
+    // loaded into registers while the current group is computed: lane value j
+    // is entry idx = lane + 64 j of the group's EW x 60 block
+    constexpr int NL = (EW * 60 + 63) / 64;
+    double px[NL], pu[NL], ps = 0.0;
+    uint32_t pm[NL];
+    auto issue = [&](uint32_t grp) {
+        const uint32_t first = grp * EW, n_here = min((uint32_t)EW, n_slots - first);
+        const uint64_t slot0 = (uint64_t)slot_begin + first;
+#pragma unroll
+        for (int j = 0; j < NL; j++) {
+            const uint32_t idx = lane + 64u * j, e = idx / 60u;
+            const bool ok = idx < EW * 60u && e < n_here;
+            // branch-free: entries outside the group load the group's first entry
+            const uint64_t src = (slot0 + (ok ? e : 0u)) * 60u + (ok ? idx - 60u * e : 0u);
+            const double xv = xe[src];
+            const uint32_t mv = DIAG ? 0u : map[src];
+            px[j] = ok ? xv : 0.0;
+            pm[j] = ok ? mv : 0u;
+        }
+        const uint32_t e = lane / QP;
+        const double sv = scale[slot0 + min(e, n_here - 1u)];
+        ps = e < n_here ? sv : 0.0;
+    };
+    auto gather = [&]() {
+#pragma unroll
+        for (int j = 0; j < NL; j++) {
+            double v = 0.0;
+            if constexpr (!DIAG) {
+                v = u[pm[j] & kMask];  // entries outside the group read u[0]; their elements are not stored
+                if (MASKED && (pm[j] & kFlag)) v = 0.0;
+            }
+            pu[j] = v;
+        }
+    };
+
+    uint32_t grp = blockIdx.x * 4u + wv;  // uniform per wave
+    if (grp < n_groups) {
+        issue(grp);
+        gather();
+    }
+    for (; grp < n_groups; grp += gridDim.x * 4u) {
         const uint32_t first = grp * EW;
         const uint32_t n_here = min((uint32_t)EW, n_slots - first);
         const uint64_t slot0 = (uint64_t)slot_begin + first;
+        const uint32_t next = grp + gridDim.x * 4u;
 
-        // cooperative load: element e's node a at W[e*kElemRow + 6a]: x y z, then u
-        for (uint32_t idx = lane; idx < EW * 60u; idx += 64u) {
-            const uint32_t e = idx / 60u, r = idx - e * 60u, a = r / 3u, k = r - 3u * a;
-            double xv = 0.0, uv = 0.0;
-            if (e < n_here) {
-                const uint64_t src = (slot0 + e) * 60u + r;
-                xv = xe[src];
-                if constexpr (!DIAG) {
-                    const uint32_t mm = map[src];
-                    uv = (MASKED && (mm & kFlag)) ? 0.0 : u[mm & kMask];
-                }
+        // element e's node a at W[e*kElemRow + 6a]: x y z, then u
+#pragma unroll
+        for (int j = 0; j < NL; j++) {
+            const uint32_t idx = lane + 64u * j;
+            if (idx < EW * 60u) {
+                const uint32_t e = idx / 60u, r = idx - e * 60u, a = r / 3u, k = r - 3u * a;
+                W[e * kElemRow + 6u * a + k] = px[j];
+                W[e * kElemRow + 6u * a + 3u + k] = pu[j];
             }
-            W[e * kElemRow + 6u * a + k] = xv;
-            W[e * kElemRow + 6u * a + 3u + k] = uv;
         }
+        const double sc_here = ps;
         lds_sync_wave();
+        if (next < n_groups) issue(next);
 
         // phase 1
         {
@@ -241,7 +269,7 @@ __global__ __launch_bounds__(256) void free_hex20_kernel(const double* __restric
             }
             double inv[3][3];
             const double det = jac_inv_r(J, inv);
-            const double w = e < n_here ? det * wq[q] * scale[slot0 + e] : 0.0;
+            const double w = e < n_here ? det * wq[q] * sc_here : 0.0;
             double S[kH20Slot];
             if constexpr (DIAG) {
 #pragma unroll
@@ -277,6 +305,7 @@ __global__ __launch_bounds__(256) void free_hex20_kernel(const double* __restric
             for (int j = 0; j < kH20Slot; j += 2) *reinterpret_cast<double2*>(slotp + j) = make_double2(S[j], S[j + 1]);
         }
         lds_sync_wave();
+        if (next < n_groups) gather();
 
         // phase 2: f[e] = sum_q dN_q(a) . P_q(:, i) of element e
         double f[EW];
@@ -297,19 +326,11 @@ __global__ __launch_bounds__(256) void free_hex20_kernel(const double* __restric
                 f[e] += d0 * p01.x + d1 * p01.y + d2 * p2;
             }
         };
-        if constexpr (QP == 8) {
+        for (uint32_t q = 0; q < nq; q++) {
+            const double2 t01 = *reinterpret_cast<const double2*>(tab + q * kTabRow + 4 * pa);
+            const double t2 = tab[q * kTabRow + 4 * pa + 2];
 #pragma unroll
-            for (uint32_t e = 0; e < EW; e++)
-#pragma unroll
-                for (uint32_t q = 0; q < 8; q++)
-                    if (q < nq) point(e, q, dnr[q][0], dnr[q][1], dnr[q][2]);
-        } else {
-            for (uint32_t q = 0; q < nq; q++) {
-                const double2 t01 = *reinterpret_cast<const double2*>(tab + q * kTabRow + 4 * pa);
-                const double t2 = tab[q * kTabRow + 4 * pa + 2];
-#pragma unroll
-                for (uint32_t e = 0; e < EW; e++) point(e, q, t01.x, t01.y, t2);
-            }
+            for (uint32_t e = 0; e < EW; e++) point(e, q, t01.x, t01.y, t2);
         }
         if (lane < 60) {
 #pragma unroll

@@ -19,7 +19,12 @@
  * Differences: Maps takes (rank, size) instead of an MPI communicator (the
  * halo transport belongs to the caller, see amx_maps_get_ghost_requests);
  * failures throw par::amx_error carrying the par::Error code instead of
- * exiting; methods 1/2 report NOT_IMPLEMENTED.  The PETSc shell hook
+ * exiting; method 1 reports NOT_IMPLEMENTED.  Method 2 (FREE) runs on the GPU
+ * in two forms: set_element_function (built-in element kinds recomputed
+ * in-kernel from coordinates at every matvec) or the reference's
+ * set_element_matrix_function callback, which is host code: it is evaluated
+ * once per element at finalize and its matrices are stored (the operator then
+ * runs as method 3, with the callback's values).  The PETSc shell hook
  * (get_matrix) is compiled only with BUILD_WITH_PETSC.
  */
 #ifndef AMAT_AMD_AMATFREE_HPP
@@ -167,7 +172,26 @@ template <typename DT, typename GI, typename LI> class aMatFree : public aMat<aM
 
     Error set_matfree_type_impl(MATFREE_TYPE t)
     {
+        m_type = t;
+        // FREE without a built-in element kind is resolved at finalize (callback form)
+        if (t == MATFREE_TYPE::FREE && !m_kind_set) return Error::SUCCESS;
         detail::check(amx_op_set_matfree_type(m_h, (int)t), "set_matfree_type");
+        return Error::SUCCESS;
+    }
+    /* aMat::set_element_matrix_function (include/aMat.hpp:114-116): eMat(eid,
+     * ke, xe) fills the element matrix (row-major) and the node coordinates */
+    Error set_element_matrix_function(void (*eMat)(LI, DT*, DT*))
+    {
+        m_eMat = eMat;
+        return Error::SUCCESS;
+    }
+    /* method 2 on the GPU: K_e = scale[e] * ke(xe[e]) recomputed in-kernel for
+     * the built-in kinds (amx_element_kind); xe: n_elems x 3 x nodes */
+    Error set_element_function(int kind, const DT* xe, const DT* scale, DT E, DT nu, unsigned ngt)
+    {
+        detail::check(amx_op_set_element_function(m_h, kind, xe, scale, E, nu, ngt), "set_element_function");
+        m_kind_set = true;
+        m_type = MATFREE_TYPE::FREE;
         return Error::SUCCESS;
     }
     /* aMatFree::set_num_streams (include/aMatFree.hpp:230-237) */
@@ -203,6 +227,22 @@ template <typename DT, typename GI, typename LI> class aMatFree : public aMat<aM
     }
     Error finalize_impl()
     {
+        if (m_type == MATFREE_TYPE::FREE && !m_kind_set) {
+            if (!m_eMat)
+                throw amx_error(AMX_INVALID_STATE, "finalize: FREE needs set_element_matrix_function or "
+                                                   "set_element_function");
+            const LI ne = m_maps.get_NumElems();
+            const LI* dpe = m_maps.get_DofsPerElem();
+            std::vector<DT> ke, xe;
+            for (LI eid = 0; eid < ne; eid++) {
+                ke.assign((size_t)dpe[eid] * dpe[eid], DT(0));
+                xe.assign((size_t)3 * dpe[eid], DT(0));  // at most 3 coordinates per DoF
+                m_eMat(eid, ke.data(), xe.data());
+                detail::check(amx_op_set_element_matrix(m_h, eid, ke.data(), AMX_ROW_MAJOR, 0, 0, 1),
+                              "set_element_matrix_function");
+            }
+            detail::check(amx_op_set_matfree_type(m_h, AMX_MATFREE_GPU_PURE), "set_matfree_type");
+        }
         detail::check(amx_op_finalize(m_h), "finalize");
         return Error::SUCCESS;
     }
@@ -293,6 +333,9 @@ template <typename DT, typename GI, typename LI> class aMatFree : public aMat<aM
     Maps<DT, GI, LI>& m_maps;
     amx_op* m_h = nullptr;
     std::vector<double> m_buf;
+    MATFREE_TYPE m_type = MATFREE_TYPE::GPU_PURE;
+    bool m_kind_set = false;
+    void (*m_eMat)(LI, DT*, DT*) = nullptr;
 };
 
 }  // namespace par

@@ -92,12 +92,98 @@ static int run_case(unsigned Nex, unsigned Ney, unsigned Nez)
     return (d1 == 0.0 && d2 == 0.0) ? 0 : 1;
 }
 
+// method 2 (FREE) the way examples/src/example07a.cpp:54-70 sets it up: an
+// element function reads the application's mesh and fills ke and xe
+static struct {
+    unsigned Nex, Ney, Nez;
+    std::vector<std::vector<unsigned>> map;
+} g_app;
+
+static void node_xyz(unsigned n, double* x)
+{
+    const unsigned sx = g_app.Nex + 1, sy = (g_app.Nex + 1) * (g_app.Ney + 1);
+    // a smooth distortion keeps the elements distinct (and the Jacobians positive)
+    const double X = (double)(n % sx) / g_app.Nex, Y = (double)((n / sx) % (g_app.Ney + 1)) / g_app.Ney,
+                 Z = (double)(n / sy) / g_app.Nez;
+    x[0] = X + 0.03 * std::sin(3.0 * Y + Z);
+    x[1] = Y + 0.03 * std::sin(2.0 * Z + X);
+    x[2] = Z + 0.03 * std::sin(2.5 * X + Y);
+}
+
+static void compute_elem_mat(unsigned eid, double* ke, double* xe)
+{
+    for (int q = 0; q < 8; q++) node_xyz(g_app.map[eid][q], xe + 3 * q);
+    par::detail::check(amx_fem_ke_hex8(xe, 2, ke), "ke_hex8");
+}
+
+static int run_free_case(unsigned Nex, unsigned Ney, unsigned Nez)
+{
+    using Maps = par::Maps<double, unsigned long, unsigned int>;
+    using AFM = par::aMatFree<double, unsigned long, unsigned int>;
+    g_app.Nex = Nex;
+    g_app.Ney = Ney;
+    g_app.Nez = Nez;
+    const unsigned nelem = Nex * Ney * Nez, nnode = (Nex + 1) * (Ney + 1) * (Nez + 1);
+    const unsigned sx = Nex + 1, sy = (Nex + 1) * (Ney + 1);
+    g_app.map.assign(nelem, std::vector<unsigned>(8));
+    std::vector<unsigned*> mapp(nelem);
+    std::vector<unsigned> dpe(nelem, 8), flat, order(nelem);
+    for (unsigned k = 0; k < Nez; k++)
+        for (unsigned j = 0; j < Ney; j++)
+            for (unsigned i = 0; i < Nex; i++) {
+                const unsigned e = Nex * Ney * k + Nex * j + i, n0 = i + j * sx + k * sy;
+                const unsigned c[8] = {n0, n0 + 1, n0 + sx + 1, n0 + sx, n0 + sy, n0 + sy + 1, n0 + sy + sx + 1,
+                                       n0 + sy + sx};
+                for (int q = 0; q < 8; q++) g_app.map[e][q] = c[q];
+                mapp[e] = g_app.map[e].data();
+                flat.insert(flat.end(), c, c + 8);
+                order[e] = e;
+            }
+    std::vector<unsigned long> l2g(nnode);
+    for (unsigned i = 0; i < nnode; i++) l2g[i] = i;
+    std::vector<unsigned long> cons;
+    std::vector<double> vals;
+    Maps maps;
+    maps.set_map(nelem, mapp.data(), dpe.data(), nnode, l2g.data(), 0, nnode - 1, nnode);
+    maps.set_bdr_map(cons.data(), vals.data(), 0);
+
+    // (a) the reference's callback form
+    AFM A(maps, par::BC_METH::BC_IMATRIX);
+    A.set_matfree_type(par::MATFREE_TYPE::FREE);
+    A.set_element_matrix_function(&compute_elem_mat);
+    A.finalize();
+    // (b) the in-kernel form on the same coordinates
+    std::vector<double> xall((size_t)nelem * 24), Kall((size_t)nelem * 64);
+    for (unsigned e = 0; e < nelem; e++) compute_elem_mat(e, &Kall[(size_t)e * 64], &xall[(size_t)e * 24]);
+    AFM B(maps, par::BC_METH::BC_IMATRIX);
+    B.set_matfree_type(par::MATFREE_TYPE::FREE);
+    B.set_element_function(AMX_ELEM_HEX8_LAPLACE, xall.data(), nullptr, 1.0, 0.3, 2);
+    B.finalize();
+
+    std::vector<double> u(nnode), va(nnode), vb(nnode), vref(nnode);
+    for (unsigned i = 0; i < nnode; i++) u[i] = std::sin(0.37 * i) + 0.1 * std::cos(1.3 * i);
+    A.matvec(va.data(), u.data(), true);
+    B.matvec(vb.data(), u.data(), true);
+    orc_matvec_ghosted_omp(nelem, dpe.data(), flat.data(), Kall.data(), 0, order.data(), nelem, nullptr, 0, u.data(),
+                           vref.data(), nnode, 1);
+    double da = 0, db = 0, nr = 0;
+    for (unsigned i = 0; i < nnode; i++) {
+        da = std::fmax(da, std::fabs(va[i] - vref[i]));
+        db = std::fmax(db, std::fabs(vb[i] - vref[i]));
+        nr = std::fmax(nr, std::fabs(vref[i]));
+    }
+    std::printf("dropin free_callback_%ux%ux%u maxdiff_matvec=%.3e\n", Nex, Ney, Nez, da);
+    std::printf("dropin free_inkernel_%ux%ux%u relmax=%.3e\n", Nex, Ney, Nez, db / nr);
+    return (da == 0.0 && db <= 1e-13 * nr) ? 0 : 1;
+}
+
 int main()
 {
     int bad = 0;
     try {
         bad |= run_case(4, 5, 6);
         bad |= run_case(16, 16, 16);
+        bad |= run_free_case(7, 6, 5);
         // error path: methods 1/2 belong to the reference's CPU loops
         par::Maps<double, unsigned long, unsigned int> m;
         unsigned row[8] = {0, 1, 2, 3, 4, 5, 6, 7}, *rows[1] = {row}, d8[1] = {8};

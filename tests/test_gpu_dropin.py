@@ -1,6 +1,7 @@
 """The header-only C++ drop-in (include/amat_amd/aMatFree.hpp) used the way
-examples/src/example03a.cpp uses the reference: a compiled C++ program runs
-the GPU operator and checks it bit-exact against the oracle."""
+examples/src/example03a.cpp uses the reference (and example07a.cpp sets up
+method 2): a compiled C++ program runs the GPU operator and checks it against
+the oracle."""
 import subprocess
 
 import pytest
@@ -18,3 +19,7 @@ def test_cpp_dropin_example():
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.count("maxdiff_matvec=0.000e+00 maxdiff_mf=0.000e+00") == 2
     assert "error-path code=11" in r.stdout
+    # method 2: the callback form is stored (bit-exact), the in-kernel form within 1e-13 of max |v|
+    assert "dropin free_callback_7x6x5 maxdiff_matvec=0.000e+00" in r.stdout
+    rel = float(r.stdout.split("free_inkernel_7x6x5 relmax=")[1].split()[0])
+    assert rel <= 1e-13
