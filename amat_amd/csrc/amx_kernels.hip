@@ -281,9 +281,22 @@ __device__ __forceinline__ void gather_body(const uint32_t* __restrict__ off, co
                                             const uint8_t* __restrict__ flag, const double* __restrict__ coef,
                                             VT* __restrict__ v, uint32_t d)
 {
-    const uint32_t b = off[d], e = off[d + 1];
+    const uint32_t b = off[d], cnt = off[d + 1] - b;
     VT s = VT(0);
-    for (uint32_t q = b; q < e; ++q) s += ve[pos[q]];
+    // chunks of 8 entries: all index loads, then all row loads, in flight
+    // together; the sum keeps the list order (entries past the end re-read
+    // the last one and are not added)
+    for (uint32_t q0 = 0; q0 < cnt; q0 += 8) {
+        uint32_t p[8];
+        VT x[8];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; j++) p[j] = pos[b + min(q0 + j, cnt - 1)];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; j++) x[j] = ve[p[j]];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; j++)
+            if (q0 + j < cnt) s += x[j];
+    }
     if constexpr (BC == 1) {
         if (flag[d]) s = u[d];
     } else if constexpr (BC == 2) {
