@@ -64,8 +64,11 @@ class PREC(enum.IntFlag):
     BF16 = 4
 
 
-# refinement option: inner CG vectors in FP64 (or-ed into ir_cg's precision)
+# refinement options (or-ed into ir_cg's precision): inner CG vectors in FP64;
+# the reliable-update form (one unrestarted low-precision recurrence whose
+# residual is replaced by the FP64 true residual; inner_rtol = threshold)
 IR_F64_VECTORS = 0x100
+IR_RELIABLE = 0x200
 
 
 class ELEM(enum.IntEnum):

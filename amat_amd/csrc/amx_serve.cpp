@@ -18,7 +18,8 @@
 // Methods: FP64 Jacobi-PCG (the KSPCG + PCJACOBI solve of
 // examples/include/solve.hpp:45-65), or FP64 iterative refinement whose inner
 // Jacobi-PCG iterates on the FP32 or BF16 element matrices with FP32 vectors
-// (outer residual, scalars and penalty rows in FP64).
+// (outer residual, scalars and penalty rows in FP64), restarted per outer
+// step or (AMX_IR_RELIABLE) one recurrence with reliable updates.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -45,6 +46,7 @@ struct Job {
     int64_t id = 0;
     amx_op* h = nullptr;
     int method = AMX_PREC_FP64;  // FP64 CG, or IR with FP32 / BF16 inner storage
+    bool reliable = false;       // IR form: reliable updates instead of restarts
     double rtol = 1e-10, inner_rtol = 1e-2;
     int max_it = 10000, max_outer = 50;
     int slot = -1;
@@ -56,7 +58,7 @@ struct Job {
     DevBuf b, x, dinv, ve, partials;  // FP64 rhs, solution, 1/diag; element rows; reduction partials
     DevBuf r, z, p, q;                // iteration vectors
     DevBuf xi_buf;                    // IR: inner solution (FP32); CG: unused (x iterates)
-    DevBuf r64, q64, b32;             // IR: FP64 residual, A x, scaled residual (FP32)
+    DevBuf r64, q64, b32;             // IR: FP64 residual, A x, scaled residual (FP32; restarted form)
     void* xi = nullptr;               // iteration solution vector
     uint64_t bytes = 0;
     // progress
@@ -107,8 +109,16 @@ uint64_t job_bytes(const Op& op, int method)
     const uint64_t n = std::max<uint32_t>(op.n_total, 1);
     const uint64_t ve = std::max<uint64_t>(op.ve_rows, 1) * 8;
     const uint64_t partials = 3 * 1024 * 8;
-    if (method == AMX_PREC_FP64) return 7 * n * 8 + ve + partials;  // b x dinv r z p q
-    return 5 * n * 8 + 6 * n * 4 + ve + partials;                  // b x dinv r64 q64 + r z p q xi b32
+    if (method == AMX_PREC_FP64) return 7 * n * 8 + ve + partials;           // b x dinv r z p q
+    if (method & AMX_IR_RELIABLE) return 5 * n * 8 + 5 * n * 4 + ve + partials;  // b x dinv r64 q64 + r z p q xs
+    return 5 * n * 8 + 6 * n * 4 + ve + partials;                           // b x dinv r64 q64 + r z p q xi b32
+}
+
+bool valid_method(int method)
+{
+    const int base = method & ~AMX_IR_RELIABLE;
+    if (method & AMX_IR_RELIABLE) return base == AMX_PREC_FP32 || base == AMX_PREC_BF16;
+    return base == AMX_PREC_FP64 || base == AMX_PREC_FP32 || base == AMX_PREC_BF16;
 }
 
 template <typename T>
@@ -245,6 +255,31 @@ void outer_update(amx_server& S, Job& j)
                   s);
 }
 
+// reliable updates: x += xs, r = b - A x (FP64 operator) replaces the
+// recurrence residual, p = z + beta p continues the direction (the sequence of
+// the single-solve form)
+void replace_residual(amx_server& S, Job& j)
+{
+    hipStream_t s = S.stream;
+    Op& op = j.h->op;
+    amx::ru_flush(false, j.x.as<double>(), j.xi, j.n, s);
+    amx::apply(op, KType::F64, true, op.bc_operator, j.x.p, j.q64.p, s, nullptr, j.ve.p);
+    amx::ru_replace(false, false, j.b.as<double>(), j.q64.as<double>(), j.r.p, j.z.p, j.p.p, j.dinv.as<double>(), j.n,
+                    j.partials.as<double>(), S.it_sc(j), s);
+    amx::cg_p(false, j.p.p, j.z.p, j.n, S.it_sc(j), s);
+}
+
+// true residual ||b - A x|| of a reliable-update job stopped by a breakdown
+void final_residual64(amx_server& S, Job& j)
+{
+    hipStream_t s = S.stream;
+    Op& op = j.h->op;
+    amx::ru_flush(false, j.x.as<double>(), j.xi, j.n, s);
+    amx::apply(op, KType::F64, true, op.bc_operator, j.x.p, j.q64.p, s, nullptr, j.ve.p);
+    amx::residual(j.b.as<double>(), j.q64.as<double>(), j.r64.as<double>(), j.n, j.partials.as<double>(),
+                  S.out_sc(j), s);
+}
+
 // FP64 CG after convergence: true residual ||b - A x|| into the outer scalars
 void final_residual(amx_server& S, Job& j)
 {
@@ -277,7 +312,20 @@ void advance(amx_server& S)
         case kIter:
             if (!it.done) break;
             j.breakdown = j.breakdown || it.breakdown;
-            if (j.method == AMX_PREC_FP64) {
+            if (j.reliable) {
+                j.inner_total = (int)it.iter;
+                j.outer = (int)it.n_upd;
+                if (it.breakdown) {
+                    final_residual64(S, j);
+                    j.phase = kFinalPending;
+                } else if (it.state == 2) {  // decided on the true residual
+                    j.rel = it.bb > 0 ? std::sqrt(it.rr / it.bb) : 0.0;
+                    j.converged = j.rel <= j.rtol * 1.0000001 || it.bb == 0.0;
+                    finish(S, j);
+                } else {
+                    replace_residual(S, j);  // the job runs on in the next step unless final
+                }
+            } else if (j.method == AMX_PREC_FP64) {
                 j.cg_iters = (int)it.iter;
                 final_residual(S, j);
                 j.phase = kFinalPending;
@@ -356,8 +404,8 @@ int amx_server_job_bytes(const amx_op* h, int method, uint64_t* bytes)
 {
     return guarded(__func__, [&] {
         AMX_REQUIRE(h && bytes, AMX_INVALID_ARGUMENT, "NULL argument");
-        AMX_REQUIRE(method == AMX_PREC_FP64 || method == AMX_PREC_FP32 || method == AMX_PREC_BF16,
-                    AMX_INVALID_ARGUMENT, "method: AMX_PREC_FP64 (CG) or AMX_PREC_FP32 / AMX_PREC_BF16 (IR)");
+        AMX_REQUIRE(valid_method(method), AMX_INVALID_ARGUMENT,
+                    "method: AMX_PREC_FP64 (CG) or AMX_PREC_FP32 / AMX_PREC_BF16 (IR), optionally | AMX_IR_RELIABLE");
         *bytes = job_bytes(h->op, method);
     });
 }
@@ -367,8 +415,10 @@ int amx_server_submit(amx_server* S, amx_op* h, int method, const double* b_host
 {
     return guarded(__func__, [&] {
         AMX_REQUIRE(S && h && b_host && job_id, AMX_INVALID_ARGUMENT, "NULL argument");
-        AMX_REQUIRE(method == AMX_PREC_FP64 || method == AMX_PREC_FP32 || method == AMX_PREC_BF16,
-                    AMX_INVALID_ARGUMENT, "method: AMX_PREC_FP64 (CG) or AMX_PREC_FP32 / AMX_PREC_BF16 (IR)");
+        AMX_REQUIRE(valid_method(method), AMX_INVALID_ARGUMENT,
+                    "method: AMX_PREC_FP64 (CG) or AMX_PREC_FP32 / AMX_PREC_BF16 (IR), optionally | AMX_IR_RELIABLE");
+        const bool reliable = (method & AMX_IR_RELIABLE) != 0;
+        const int base = method & ~AMX_IR_RELIABLE;
         AMX_REQUIRE(rtol > 0.0 && max_it > 0 && max_outer > 0, AMX_INVALID_ARGUMENT, "bad tolerance or limits");
         AMX_REQUIRE(method == AMX_PREC_FP64 || (inner_rtol > 0.0 && inner_rtol < 1.0), AMX_INVALID_ARGUMENT,
                     "inner_rtol must be in (0, 1)");
@@ -379,7 +429,7 @@ int amx_server_submit(amx_server* S, amx_op* h, int method, const double* b_host
         AMX_REQUIRE(op.device == S->device, AMX_INVALID_ARGUMENT, "operator on another device");
         AMX_REQUIRE(op.scatter == AMX_SCATTER_DETERMINISTIC, AMX_NOT_IMPLEMENTED,
                     "served solves use the deterministic scatter");
-        AMX_REQUIRE(op.has(method), AMX_INVALID_STATE, "the method's storage was not built at finalize");
+        AMX_REQUIRE(op.has(base), AMX_INVALID_STATE, "the method's storage was not built at finalize");
         for (auto& g : op.groups)
             AMX_REQUIRE(amx::batched_n(g.n), AMX_NOT_IMPLEMENTED,
                         "batched launches need a tile kernel for n = " + std::to_string(g.n));
@@ -390,14 +440,15 @@ int amx_server_submit(amx_server* S, amx_op* h, int method, const double* b_host
         Job& j = *J;
         j.id = S->next_id++;
         j.h = h;
-        j.method = method;
+        j.method = base;
+        j.reliable = reliable;
         j.rtol = rtol;
         j.inner_rtol = inner_rtol;
         j.max_it = max_it;
         j.max_outer = max_outer;
         j.n = op.n_total;
-        j.f64iter = method == AMX_PREC_FP64;
-        j.kt = method == AMX_PREC_FP64 ? KType::F64 : (method == AMX_PREC_FP32 ? KType::F32 : KType::BF16);
+        j.f64iter = base == AMX_PREC_FP64;
+        j.kt = base == AMX_PREC_FP64 ? KType::F64 : (base == AMX_PREC_FP32 ? KType::F32 : KType::BF16);
         const size_t n = std::max<uint32_t>(j.n, 1);
         const size_t vs = j.f64iter ? 8 : 4;
         j.b.alloc(n * 8);
@@ -408,7 +459,7 @@ int amx_server_submit(amx_server* S, amx_op* h, int method, const double* b_host
         for (DevBuf* v : {&j.r, &j.z, &j.p, &j.q}) v->alloc(n * vs);
         if (!j.f64iter) {
             j.xi_buf.alloc(n * 4);
-            j.b32.alloc(n * 4);
+            if (!reliable) j.b32.alloc(n * 4);
             j.r64.alloc(n * 8);
             j.q64.alloc(n * 8);
             j.xi = j.xi_buf.p;
@@ -422,7 +473,16 @@ int amx_server_submit(amx_server* S, amx_op* h, int method, const double* b_host
         AMX_HIP(hipMemcpyAsync(j.b.p, b_host, (size_t)j.n * 8, hipMemcpyHostToDevice, s));
         amx::cg_reset(S->out_sc(j), rtol, (uint32_t)max_outer, s);  // zeroed outer scalars (reduction counter)
         amx::build_dinv(op, j.dinv.as<double>(), j.x.as<double>(), s);  // x as scratch (zeroed below)
-        if (j.f64iter) {
+        if (reliable) {
+            // the single-solve start: x = xs = 0, the recurrence residual from b
+            amx::cg_reset(S->it_sc(j), rtol, (uint32_t)max_it, s, inner_rtol, (uint32_t)max_outer);
+            amx::launch_fill(j.x.p, j.n, true, 0.0, s);
+            amx::launch_fill(j.xi, j.n, false, 0.0, s);
+            amx::launch_fill(j.q64.p, j.n, true, 0.0, s);
+            amx::ru_replace(false, true, j.b.as<double>(), j.q64.as<double>(), j.r.p, j.z.p, j.p.p,
+                            j.dinv.as<double>(), j.n, j.partials.as<double>(), S->it_sc(j), s);
+            j.phase = kIter;
+        } else if (j.f64iter) {
             amx::cg_reset(S->it_sc(j), rtol, (uint32_t)max_it, s);
             amx::cg_init(true, j.b.p, j.dinv.as<double>(), j.x.p, j.r.p, j.z.p, j.p.p, j.n, j.partials.as<double>(),
                          S->it_sc(j), s);
@@ -490,7 +550,7 @@ int amx_server_result(amx_server* S, int64_t job_id, double* x_host, amx_solve_s
             stats->iterations = j.method == AMX_PREC_FP64 ? j.cg_iters : j.inner_total;
             stats->outer_iterations = j.outer;
             stats->converged = j.converged ? 1 : 0;
-            stats->precision = j.method;
+            stats->precision = j.method | (j.reliable ? AMX_IR_RELIABLE : 0);
             stats->rel_residual = j.rel;
             stats->seconds = std::chrono::duration<double>(j.t_done - j.t_submit).count();
         }

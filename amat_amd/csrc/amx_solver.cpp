@@ -5,6 +5,7 @@
 // The iteration runs without host round trips: scalars stay on the device and
 // the host only polls the converged flag every few iterations (launches after
 // convergence are no-ops, the matvec kernels included).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 
@@ -53,11 +54,14 @@ void ensure_ws(Op& op)
     }
 }
 
-void set_scalars(Op& op, CgScalars* dsc, double rtol, int max_it, hipStream_t s)
+void set_scalars(Op& op, CgScalars* dsc, double rtol, int max_it, hipStream_t s, double delta = 0.0,
+                 int max_upd = 0)
 {
     CgScalars h{};
     h.tol2 = rtol * rtol;
     h.max_it = (uint32_t)max_it;
+    h.d2 = delta * delta;
+    h.max_upd = (uint32_t)max_upd;
     h.counter = 0;
     *op.host_sc = h;
     AMX_HIP(hipMemcpyAsync(dsc, op.host_sc, sizeof(CgScalars), hipMemcpyHostToDevice, s));
@@ -97,6 +101,67 @@ CgScalars pcg(Op& op, KType kt, bool f64, const void* b, void* x, void* r, void*
         launched += (int)chunk;
         const CgScalars sc = read_scalars(op, dsc, s);
         if (sc.done || launched >= max_it) return sc;
+    }
+}
+
+// Mixed-precision PCG with reliable updates (residual replacement: Sleijpen
+// and van der Vorst, "Reliable updated residuals in hybrid Bi-CG methods",
+// Computing 56 (1996); the form GPU mixed-precision solvers use).  The CG
+// recurrence runs with the operator of storage kt and vectors of type f64v
+// from one start, never restarted.  The device stops it (scalars' state 1)
+// when its ||r|| has fallen below delta times its largest value since the last
+// replacement, or it reports convergence or the iteration limit; the host then
+// flushes the recurrence's increment into the FP64 solution, replaces the
+// recurrence residual by the true b - A x (FP64 operator), continues the
+// direction update p = z + beta p and lets it run again unless the true
+// residual has converged or a limit is reached (state 2).  A replacement that
+// finds the true residual norm more than 4x the recurrence's restarts the
+// direction (beta = 0): then the low-precision operator's error, not the
+// recurrence, limits progress, and the step is a restarted refinement step.
+// A phase also ends after twice the previous phase's length (64 iterations at
+// least), so that a stalled recurrence is replaced too.  The stop is decided on the device, so the
+// iterates do not depend on how often the host polls.
+CgScalars ru_pcg(Op& op, KType kt, bool f64v, const double* b, double* x, void* xs, void* rs, void* zs, void* ps,
+                 void* qs, double rtol, double delta, int max_it, int max_upd, CgScalars* dsc, hipStream_t s,
+                 int& n_upd, bool& true_rr)
+{
+    const uint32_t n = op.n_total;
+    double* partials = op.ws_partials.as<double>();
+    const double* dinv = op.ws_dinv.as<double>();
+    double* q64 = op.ws_q.as<double>();
+    set_scalars(op, dsc, rtol, max_it, s, delta, max_upd);
+    launch_fill(x, n, true, 0.0, s);
+    launch_fill(xs, n, f64v, 0.0, s);
+    launch_fill(q64, n, true, 0.0, s);
+    ru_replace(f64v, true, b, q64, rs, zs, ps, dinv, n, partials, dsc, s);
+    CgScalars sc = read_scalars(op, dsc, s);
+    n_upd = 0;
+    true_rr = true;
+    if (sc.done) return sc;
+    const uint32_t chunk = n > (1u << 21) ? 2 : (n > (1u << 17) ? 4 : 8);
+    const uint32_t* skip = &dsc->done;
+    for (;;) {
+        for (uint32_t k = 0; k < chunk; k++) {
+            apply(op, kt, f64v, op.bc_operator, ps, qs, s, skip);
+            cg_pq(f64v, ps, qs, n, partials, dsc, s);
+            cg_update(f64v, xs, rs, zs, ps, qs, dinv, n, partials, dsc, s);
+            cg_p(f64v, ps, zs, n, dsc, s);
+        }
+        sc = read_scalars(op, dsc, s);
+        if (!sc.done) continue;
+        if (sc.breakdown) {
+            ru_flush(f64v, x, xs, n, s);
+            true_rr = false;
+            return sc;
+        }
+        // state 1: replace the residual, continue the direction
+        ru_flush(f64v, x, xs, n, s);
+        apply(op, KType::F64, true, op.bc_operator, x, q64, s);
+        ru_replace(f64v, false, b, q64, rs, zs, ps, dinv, n, partials, dsc, s);
+        cg_p(f64v, ps, zs, n, dsc, s);  // a no-op once final
+        sc = read_scalars(op, dsc, s);
+        n_upd = (int)sc.n_upd;
+        if (sc.done) return sc;
     }
 }
 
@@ -146,9 +211,11 @@ int amx_ir_cg_solve_dev(amx_op* h, int precision, const double* b, double* x, do
     return guarded(__func__, [&] {
         AMX_REQUIRE(h && b && x, AMX_INVALID_ARGUMENT, "NULL argument");
         const bool v64 = (precision & AMX_IR_F64_VECTORS) != 0;
-        precision &= ~AMX_IR_F64_VECTORS;
+        const bool reliable = (precision & AMX_IR_RELIABLE) != 0;
+        const int flags = precision & (AMX_IR_F64_VECTORS | AMX_IR_RELIABLE);
+        precision &= ~(AMX_IR_F64_VECTORS | AMX_IR_RELIABLE);
         AMX_REQUIRE(precision == AMX_PREC_FP32 || precision == AMX_PREC_BF16, AMX_INVALID_ARGUMENT,
-                    "inner precision must be FP32 or BF16 (optionally | AMX_IR_F64_VECTORS)");
+                    "inner precision must be FP32 or BF16 (optionally | AMX_IR_F64_VECTORS | AMX_IR_RELIABLE)");
         AMX_REQUIRE(rtol > 0.0 && inner_rtol > 0.0 && inner_rtol < 1.0 && max_outer > 0 && max_inner > 0,
                     AMX_INVALID_ARGUMENT, "bad tolerances or iteration limits");
         Op& op = h->op;
@@ -168,6 +235,32 @@ int amx_ir_cg_solve_dev(amx_op* h, int precision, const double* b, double* x, do
         amx::CgScalars* inner = outer + 1;
         const uint32_t n = op.n_total;
         double* partials = op.ws_partials.as<double>();
+        if (reliable) {
+            int n_upd = 0;
+            bool true_rr = true;
+            amx::CgScalars sc =
+                v64 ? amx::ru_pcg(op, kt, true, b, x, op.ws_ix.p, op.ws_ir.p, op.ws_iz.p, op.ws_ip.p, op.ws_iq.p, rtol,
+                                  inner_rtol, max_inner, max_outer, outer, s, n_upd, true_rr)
+                    : amx::ru_pcg(op, kt, false, b, x, op.ws_x32.p, op.ws_r32.p, op.ws_z32.p, op.ws_p32.p,
+                                  op.ws_q32.p, rtol, inner_rtol, max_inner, max_outer, outer, s, n_upd, true_rr);
+            const bool breakdown = sc.breakdown != 0;
+            if (!true_rr) {  // stopped between replacements: measure the true residual
+                amx::apply(op, KType::F64, true, op.bc_operator, x, op.ws_q.p, s);
+                amx::residual(b, op.ws_q.as<double>(), op.ws_r.as<double>(), n, partials, outer, s);
+                sc = amx::read_scalars(op, outer, s);
+            }
+            const double rel = sc.bb > 0 ? std::sqrt(sc.rr / sc.bb) : 0.0;
+            if (stats) {
+                stats->iterations = (int32_t)sc.iter;
+                stats->outer_iterations = n_upd;
+                stats->converged = rel <= rtol * 1.0000001 || sc.bb == 0.0;
+                stats->precision = precision | flags;
+                stats->rel_residual = rel;
+                stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            }
+            AMX_REQUIRE(!breakdown, AMX_NOT_CONVERGED, "CG breakdown (low-precision operator not SPD)");
+            return;
+        }
         // x = 0, r = b (the outer scalars start zeroed: their reduction counter included)
         amx::cg_reset(outer, rtol, (uint32_t)max_outer, s);
         amx::launch_fill(x, n, true, 0.0, s);
@@ -205,7 +298,7 @@ int amx_ir_cg_solve_dev(amx_op* h, int precision, const double* b, double* x, do
             stats->iterations = total_inner;
             stats->outer_iterations = outer_it;
             stats->converged = rel <= rtol;
-            stats->precision = precision | (v64 ? AMX_IR_F64_VECTORS : 0);
+            stats->precision = precision | flags;
             stats->rel_residual = rel;
             stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         }

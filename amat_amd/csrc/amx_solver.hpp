@@ -7,10 +7,19 @@
 
 namespace amx {
 
-// Device-resident state of one PCG run.
+// Device-resident state of one PCG run.  Reliable-update runs (d2 > 0) also
+// keep rmax2, the largest recurrence ||r||^2 since the last replacement, and
+// `state`: 1 = the recurrence stopped for a replacement (its ||r|| fell below
+// sqrt(d2) of rmax, it ran twice as long as the previous phase (at least 64
+// iterations) without that, or it reports convergence or the iteration
+// limit), 2 =
+// final (decided on the true residual by the replacement, or the replacement
+// limit reached).
 struct CgScalars {
-    double rz, pq, rr, bb, alpha, beta, tol2;
-    uint32_t done, iter, max_it, breakdown, counter, pad;
+    double rz, pq, rr, bb, alpha, beta, tol2, rmax2, d2, rr_prev;  // rr_prev: true ||r||^2 at the last replacement
+    uint32_t done, iter, max_it, breakdown, counter, state;
+    uint32_t n_upd, max_upd;      // replacements done / allowed (reliable updates)
+    uint32_t upd_iter, phase_len; // iteration of the last replacement; iterations of the phase before it
 };
 
 // one job of a batched CG launch (vectors of the launch's type)
@@ -46,6 +55,12 @@ void cg_update_batched(bool f64, const CgJob* jobs, const uint32_t* prefix, uint
                        hipStream_t s);
 void cg_p_batched(bool f64, const CgJob* jobs, const uint32_t* prefix, uint32_t n_jobs, uint32_t total_blocks,
                   hipStream_t s);
-void cg_reset(CgScalars* sc, double rtol, uint32_t max_it, hipStream_t s);
+void cg_reset(CgScalars* sc, double rtol, uint32_t max_it, hipStream_t s, double delta = 0.0,
+              uint32_t max_upd = 0);
+// reliable updates: x += xs, xs = 0; and the residual replacement from Ax
+// (init: Ax = 0 at the start, also sets bb and p = z)
+void ru_flush(bool f64, double* x, void* xs, uint32_t n, hipStream_t s);
+void ru_replace(bool f64, bool init, const double* b, const double* ax, void* rs, void* zs, void* ps,
+                const double* dinv, uint32_t n, double* partials, CgScalars* sc, hipStream_t s);
 
 }  // namespace amx

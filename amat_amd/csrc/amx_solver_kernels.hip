@@ -77,6 +77,37 @@ __device__ bool reduce_partials(const double (&v)[NV], double* partials, uint32_
     return reduce_partials_n<NV>(v, partials, counter, tot, sh, blockIdx.x, gridDim.x);
 }
 
+// end of a CG update (one thread): plain runs stop on convergence or the
+// iteration limit; reliable-update runs (d2 > 0) stop there too, or when ||r||
+// has fallen below sqrt(d2) of its largest value since the last replacement,
+// or when the phase has run twice as long as the previous one (64 iterations
+// at least: a recurrence stalled by the operator's error), and leave the
+// decision to the replacement (state 1).  A stopped run keeps
+// rz and beta of the previous iteration.
+__device__ __forceinline__ void cg_update_finish(CgScalars* sc, double rz_new, double rr)
+{
+    sc->iter += 1;
+    sc->rr = rr;
+    const bool reliable = sc->d2 > 0.0;
+    if (rr <= sc->tol2 * sc->bb || sc->iter >= sc->max_it) {
+        sc->done = 1u;
+        if (reliable) sc->state = 1u;
+        return;
+    }
+    if (reliable) {
+        const double m = rr > sc->rmax2 ? rr : sc->rmax2;
+        sc->rmax2 = m;
+        const uint32_t len = sc->iter - sc->upd_iter, cap = 2u * sc->phase_len;
+        if (rr < sc->d2 * m || len >= (cap > 64u ? cap : 64u)) {
+            sc->done = 1u;
+            sc->state = 1u;
+            return;
+        }
+    }
+    sc->beta = rz_new / sc->rz;
+    sc->rz = rz_new;
+}
+
 template <typename VT>
 __global__ __launch_bounds__(kBlock) void cg_init_kernel(const VT* __restrict__ b, const double* __restrict__ dinv,
                                                         VT* __restrict__ x, VT* __restrict__ r, VT* __restrict__ z,
@@ -151,16 +182,7 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(VT* __restrict__ x, V
     }
     double tot[2];
     if (!reduce_partials<2>(acc, partials, &sc->counter, tot, sh)) return;
-    if (threadIdx.x == 0) {
-        sc->iter += 1;
-        sc->rr = tot[1];
-        if (tot[1] <= sc->tol2 * sc->bb || sc->iter >= sc->max_it) {
-            sc->done = 1u;
-        } else {
-            sc->beta = tot[0] / sc->rz;
-            sc->rz = tot[0];
-        }
-    }
+    if (threadIdx.x == 0) cg_update_finish(sc, tot[0], tot[1]);
 }
 
 template <typename VT>
@@ -290,16 +312,7 @@ __global__ __launch_bounds__(kBlock) void cg_update_batched_kernel(const CgJob* 
     }
     double tot[2];
     if (!reduce_partials_n<2>(acc, jb.partials, &sc->counter, tot, sh, lb, nblk)) return;
-    if (threadIdx.x == 0) {
-        sc->iter += 1;
-        sc->rr = tot[1];
-        if (tot[1] <= sc->tol2 * sc->bb || sc->iter >= sc->max_it) {
-            sc->done = 1u;
-        } else {
-            sc->beta = tot[0] / sc->rz;
-            sc->rz = tot[0];
-        }
-    }
+    if (threadIdx.x == 0) cg_update_finish(sc, tot[0], tot[1]);
 }
 
 template <typename VT>
@@ -318,14 +331,85 @@ __global__ __launch_bounds__(kBlock) void cg_p_batched_kernel(const CgJob* __res
     for (uint32_t i = lb * kBlock + threadIdx.x; i < n; i += nblk * kBlock) p[i] = (VT)((double)z[i] + beta * (double)p[i]);
 }
 
+// ---- reliable updates (mixed-precision CG with residual replacement) ----
+// x += xs (FP64), xs = 0
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void ru_flush_kernel(double* __restrict__ x, VT* __restrict__ xs, uint32_t n)
+{
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        x[i] += (double)xs[i];
+        xs[i] = VT(0);
+    }
+}
+
+// Replace the recurrence's residual by the true one: r = b - Ax in FP64,
+// rs = (VT) r, zs = dinv rs; ||r||^2 and rs.zs in FP64.  The finishing block
+// sets rr (true), rz, beta for the direction update that follows (rz_new /
+// rz_old, or 0 to restart the direction) and done (converged, or a limit
+// reached).  init: first call (Ax = 0): also bb = ||b||^2, p = zs, counters
+// cleared.
+template <typename VT, bool INIT>
+__global__ __launch_bounds__(kBlock) void ru_replace_kernel(const double* __restrict__ b, const double* __restrict__ ax,
+                                                           VT* __restrict__ rs, VT* __restrict__ zs,
+                                                           VT* __restrict__ ps, const double* __restrict__ dinv,
+                                                           uint32_t n, double* partials, CgScalars* sc)
+{
+    __shared__ double sh[kBlock];
+    double acc[2] = {0.0, 0.0};
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const double ri = b[i] - ax[i];
+        const VT rv = (VT)ri;
+        const VT zv = (VT)(dinv[i] * (double)rv);
+        rs[i] = rv;
+        zs[i] = zv;
+        if constexpr (INIT) ps[i] = zv;
+        acc[0] += ri * ri;
+        acc[1] += (double)rv * (double)zv;
+    }
+    double tot[2];
+    if (!reduce_partials<2>(acc, partials, &sc->counter, tot, sh)) return;
+    if (threadIdx.x == 0) {
+        if constexpr (INIT) {
+            sc->bb = tot[0];
+            sc->iter = 0;
+            sc->breakdown = 0u;
+            sc->beta = 0.0;
+            sc->n_upd = 0u;
+            sc->upd_iter = 0u;
+            sc->phase_len = 0u;
+        } else {
+            // keep the direction while the recurrence tracked the true residual
+            // (true norm at most 4x the recurrence's); otherwise the
+            // low-precision operator's error dominates and the direction
+            // restarts (beta = 0), as in a restarted refinement step
+            const bool tracked = tot[0] <= 16.0 * sc->rr;
+            sc->beta = (tracked && sc->rz != 0.0) ? tot[1] / sc->rz : 0.0;
+            sc->n_upd += 1u;
+            sc->phase_len = sc->iter - sc->upd_iter;
+            sc->upd_iter = sc->iter;
+        }
+        sc->rr_prev = tot[0];
+        sc->rr = tot[0];
+        sc->rz = tot[1];
+        sc->rmax2 = tot[0];
+        sc->done = (tot[0] <= sc->tol2 * sc->bb || tot[0] == 0.0 || sc->iter >= sc->max_it ||
+                    sc->n_upd >= sc->max_upd)
+                       ? 1u
+                       : 0u;
+        sc->state = sc->done ? 2u : 0u;
+    }
+}
+
 // restart a CG: scalars zeroed, tolerance and iteration limit set (a kernel,
 // so it can sit in a stream or a graph without host staging)
-__global__ void cg_reset_kernel(CgScalars* sc, double tol2, uint32_t max_it)
+__global__ void cg_reset_kernel(CgScalars* sc, double tol2, uint32_t max_it, double d2, uint32_t max_upd)
 {
     if (threadIdx.x != 0) return;
     CgScalars h{};
     h.tol2 = tol2;
     h.max_it = max_it;
+    h.d2 = d2;
+    h.max_upd = max_upd;
     *sc = h;
 }
 
@@ -428,9 +512,29 @@ void cg_p_batched(bool f64, const CgJob* jobs, const uint32_t* prefix, uint32_t 
     AMX_HIP(hipGetLastError());
 }
 
-void cg_reset(CgScalars* sc, double rtol, uint32_t max_it, hipStream_t s)
+void ru_flush(bool f64, double* x, void* xs, uint32_t n, hipStream_t s)
 {
-    cg_reset_kernel<<<1, 64, 0, s>>>(sc, rtol * rtol, max_it);
+    const uint32_t g = reduction_grid(n);
+    if (f64) ru_flush_kernel<double><<<g, kBlock, 0, s>>>(x, (double*)xs, n);
+    else ru_flush_kernel<float><<<g, kBlock, 0, s>>>(x, (float*)xs, n);
+    AMX_HIP(hipGetLastError());
+}
+
+void ru_replace(bool f64, bool init, const double* b, const double* ax, void* rs, void* zs, void* ps,
+                const double* dinv, uint32_t n, double* partials, CgScalars* sc, hipStream_t s)
+{
+    const uint32_t g = reduction_grid(n);
+#define AMX_RU(VT, I)                                                                                              \
+    ru_replace_kernel<VT, I><<<g, kBlock, 0, s>>>(b, ax, (VT*)rs, (VT*)zs, (VT*)ps, dinv, n, partials, sc)
+    if (f64) { if (init) AMX_RU(double, true); else AMX_RU(double, false); }
+    else { if (init) AMX_RU(float, true); else AMX_RU(float, false); }
+#undef AMX_RU
+    AMX_HIP(hipGetLastError());
+}
+
+void cg_reset(CgScalars* sc, double rtol, uint32_t max_it, hipStream_t s, double delta, uint32_t max_upd)
+{
+    cg_reset_kernel<<<1, 64, 0, s>>>(sc, rtol * rtol, max_it, delta * delta, max_upd);
     AMX_HIP(hipGetLastError());
 }
 

@@ -229,7 +229,7 @@ def estimate_job_bytes(kind: str, n: int, method: PREC) -> int:
         E, nd = 6 * n ** 3, 10
         N = (n + 1) ** 3 + 7 * n ** 3 + 3 * n * (n + 1) ** 2
     ve = E * nd * 8
-    return (7 * N * 8 if method == PREC.FP64 else 5 * N * 8 + 6 * N * 4) + ve + 3 * 1024 * 8
+    return (7 * N * 8 if int(method) == PREC.FP64 else 5 * N * 8 + 6 * N * 4) + ve + 3 * 1024 * 8
 
 
 @dataclasses.dataclass
@@ -259,7 +259,7 @@ class ServingLoop:
                  clock: Callable[[], float] = time.perf_counter):
         self.server = server
         self.cache = cache
-        self.method = PREC(method)
+        self.method = int(method)
         self.budget = int(budget_bytes)
         self.n_iter = n_iter
         self.rtol = rtol

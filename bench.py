@@ -309,8 +309,12 @@ def serve_mode(args) -> dict:
     (resident cache) before the timed streams."""
     import numpy as np
     from amat_amd import PREC, serve
+    from amat_amd.amat import IR_RELIABLE
     rates = [float(r) for r in args.serve_rates.split(",")]
-    methods = {"fp64": PREC.FP64, "fp32": PREC.FP32, "bf16": PREC.BF16}
+    # fp32 / bf16: refinement restarted per outer step (inner tolerance --inner-rtol);
+    # fp32r / bf16r: reliable updates (replacement threshold --ru-delta)
+    methods = {"fp64": PREC.FP64, "fp32": PREC.FP32, "bf16": PREC.BF16,
+               "fp32r": int(PREC.FP32) | IR_RELIABLE, "bf16r": int(PREC.BF16) | IR_RELIABLE}
     names = args.serve_methods.split(",")
     streams = {r: serve.make_requests(args.serve_requests, r, seed=args.serve_seed) for r in rates}
     keys = sorted({(q.kind, q.n) for st in streams.values() for q in st})
@@ -324,14 +328,14 @@ def serve_mode(args) -> dict:
     log(f"[serve] {len(keys)} operators resident: {cache.bytes / 2**30:.1f} GiB, built in {cache.build_seconds:.1f}s")
     res = {"metric": "C5 served CG solves/s and latency at 1e-10 relative residual", "requests": args.serve_requests,
            "rates": rates, "budget_GiB": args.serve_budget_gb, "slots": args.serve_slots, "step_iters": args.serve_iters,
-           "inner_rtol": args.inner_rtol,
+           "inner_rtol": args.inner_rtol, "ru_delta": args.ru_delta,
            "operators": len(keys), "operator_GiB": cache.bytes / 2**30, "operator_build_s": cache.build_seconds,
            "runs": []}
     for rate in rates:
         for name in names:
             server = serve.Server(max_jobs=args.serve_slots)
             loop = serve.ServingLoop(server, cache, methods[name], budget, n_iter=args.serve_iters,
-                                     inner_rtol=args.inner_rtol)
+                                     inner_rtol=args.ru_delta if name.endswith("r") else args.inner_rtol)
             last = [0.0]
 
             def prog(el, fin, run, pend):
@@ -366,7 +370,9 @@ def main() -> int:
     ap.add_argument("--mode", choices=["matvec", "precision", "batch", "serve", "free"], default="matvec")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rates", default="50,200,800")
-    ap.add_argument("--serve-methods", default="fp64,bf16,fp32")
+    ap.add_argument("--serve-methods", default="fp64,fp32r,fp32,bf16")
+    ap.add_argument("--ru-delta", type=float, default=0.1,
+                    help="reliable-update replacement threshold (methods fp32r, bf16r, --mode precision)")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--serve-slots", type=int, default=64)
     ap.add_argument("--serve-iters", type=int, default=8)

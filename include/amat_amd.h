@@ -84,9 +84,14 @@ enum amx_element_kind {
 
 /* storage precision of the element matrices (bit mask, amx_op_set_options) */
 enum amx_precision { AMX_PREC_FP64 = 1, AMX_PREC_FP32 = 2, AMX_PREC_BF16 = 4 };
-/* refinement option (or-ed into the precision argument of the refinement
- * entry points): inner CG vectors in FP64 instead of FP32 */
-enum amx_ir_flags { AMX_IR_F64_VECTORS = 0x100 };
+/* refinement options (or-ed into the precision argument of the refinement
+ * entry points): AMX_IR_F64_VECTORS, inner CG vectors in FP64 instead of
+ * FP32; AMX_IR_RELIABLE, one low-precision CG recurrence never restarted,
+ * whose residual is replaced by the FP64 true residual b - A x (and its
+ * increment flushed into the FP64 solution) each time it has fallen by
+ * inner_rtol (then the replacement threshold delta) since the last
+ * replacement; max_outer bounds the replacements, max_inner the iterations */
+enum amx_ir_flags { AMX_IR_F64_VECTORS = 0x100, AMX_IR_RELIABLE = 0x200 };
 
 /* how element vectors are summed into v */
 enum amx_scatter {
@@ -335,7 +340,8 @@ int amx_cg_solve_dev(amx_op* op, const double* b, double* x, double rtol, int ma
                      amx_solve_stats* stats, void* stream);
 /* FP64 iterative refinement: true residual, alpha/beta, inner products and the
  * penalty diagonal in FP64; inner Jacobi-PCG on the FP32 or BF16 operator to
- * inner_rtol.  precision = AMX_PREC_FP32 or AMX_PREC_BF16. */
+ * inner_rtol.  precision = AMX_PREC_FP32 or AMX_PREC_BF16, optionally or-ed
+ * with amx_ir_flags (AMX_IR_RELIABLE: the reliable-update form, see above). */
 int amx_ir_cg_solve_dev(amx_op* op, int precision, const double* b, double* x, double rtol,
                         double inner_rtol, int max_outer, int max_inner,
                         amx_solve_stats* stats, void* stream);

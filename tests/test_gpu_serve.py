@@ -3,7 +3,10 @@
 Bar: a job's solution is BIT-IDENTICAL alone, in a mix admitted together, in
 a mix where it joins while others are iterating, and to the single-solve entry
 points (amx_cg_solve_dev for FP64 CG, amx_ir_cg_solve_dev for refinement):
-the batched launches run each job's own fixed-order arithmetic.  Every served
+the batched launches run each job's own fixed-order arithmetic (the
+reliable-update form stops its recurrence on the device, so the host's poll
+schedule -- 8 iterations per step alone, 4 in the join test, the
+single-solve's own chunk -- does not change the iterates).  Every served
 solve reaches 1e-10 true relative residual (FP64, measured on the device),
 and the FP64 CG solution agrees with scipy's sparse direct solve of the
 oracle-assembled system to 1e-8.
@@ -12,12 +15,18 @@ import numpy as np
 import pytest
 
 from amat_amd import PREC, AmatError, serve
+from amat_amd.amat import IR_RELIABLE
 from amat_amd.serve import Server, build_operator, build_problem, rhs_for
 
 pytestmark = pytest.mark.gpu
 
 CASES = [("hex8", 7), ("hex20", 3), ("tet10", 3), ("hex8", 5)]
-METHODS = [PREC.FP64, PREC.FP32, PREC.BF16]
+# FP64 CG; refinement restarted per outer step; refinement with reliable
+# updates (FP32: BF16's operator error stalls the unrestarted recurrence on
+# elasticity, so BF16 reliable updates are tested on the Laplace cases)
+FP32R = int(PREC.FP32) | IR_RELIABLE
+BF16R = int(PREC.BF16) | IR_RELIABLE
+METHODS = [PREC.FP64, PREC.FP32, PREC.BF16, FP32R]
 
 
 @pytest.fixture(scope="module")
@@ -96,12 +105,24 @@ def test_matches_single_solve_entry_points(ops, torch_gpu):
         xs, ss = solve_alone(op, p, b, PREC.FP64)
         np.testing.assert_array_equal(xs, x.cpu().numpy())
         assert ss.iterations == st.iterations
-        for prec in (PREC.FP32, PREC.BF16):
+        for prec in (PREC.FP32, PREC.BF16, FP32R) + ((BF16R,) if p.name != "elasticity_hex20" else ()):
             x = torch.zeros_like(db)
             st = op.ir_cg(prec, db, x, rtol=1e-10, inner_rtol=1e-2, max_outer=60, max_inner=20000)
             xs, ss = solve_alone(op, p, b, prec)
             np.testing.assert_array_equal(xs, x.cpu().numpy())
             assert ss.iterations == st.iterations and ss.outer_iterations == st.outer_iterations
+
+
+def test_bf16_reliable_laplace_mix(ops):
+    lap = [(p, op, b) for p, op, b in ops if p.name != "elasticity_hex20"]
+    alone = [solve_alone(op, p, b, BF16R) for p, op, b in lap]
+    for x, st in alone:
+        assert st.converged and st.rel_residual <= 1e-10
+    s = Server(max_jobs=8)
+    ids = [s.submit(op, BF16R, b) for p, op, b in lap]
+    res = run_all(s, ids, {j: p.n_dofs for j, (p, _, _) in zip(ids, lap)})
+    for jid, (xa, sa) in zip(ids, alone):
+        np.testing.assert_array_equal(res[jid][0], xa)
 
 
 def test_fp64_solution_vs_direct(ops):

@@ -79,3 +79,41 @@ def test_ir_cg(torch_gpu, prec, which):
           f"(FP64 CG {s64.iterations}) relres={sir.rel_residual:.2e} err_vs_fp64={err:.2e}")
     assert sir.converged and sir.rel_residual <= 1e-10
     assert err < 1e-8
+
+
+@pytest.mark.parametrize("prec", [PREC.FP32, PREC.BF16])
+@pytest.mark.parametrize("which", ["hex8", "tet10", "hex20_penalty"])
+def test_reliable_update_cg(torch_gpu, prec, which):
+    """Mixed-precision CG with reliable updates: the low-precision recurrence is
+    never restarted, so it takes about the FP64 iteration count; convergence
+    is judged on the FP64 true residual."""
+    torch = torch_gpu
+    from amat_amd.amat import IR_RELIABLE
+    if which == "hex8":
+        p, bcop, pen = problems.poisson_hex8(12), BCOP.IDENTITY, None
+    elif which == "tet10":
+        p, bcop, pen = problems.poisson_tet10(5), BCOP.IDENTITY, None
+    else:
+        p, bcop, pen = problems.elasticity_hex20(3), BCOP.PENALTY_DIAG, 1e6
+    op = p.build_operator(aMatFree, Maps, precision=PREC.FP64 | prec, bcop=bcop, penalty=pen)
+    b = (p.rhs if p.rhs is not None else mesh.uniform(6, 6, p.n_dofs, -1, 1)).copy()
+    if bcop == BCOP.IDENTITY:
+        b[p.constrained.astype(np.int64)] = 0.0
+    db = dev(torch, b)
+    x64 = torch.zeros_like(db)
+    s64 = op.cg(db, x64, rtol=1e-10, stream=torch.cuda.current_stream())
+    xr = torch.zeros_like(db)
+    sr = op.ir_cg(int(prec) | IR_RELIABLE, db, xr, rtol=1e-10, inner_rtol=0.1, max_outer=200, max_inner=20000,
+                  stream=torch.cuda.current_stream())
+    err = rel_l2(xr.cpu().numpy(), x64.cpu().numpy())
+    print(f"RU-CG {prec.name} {which}: replacements={sr.outer_iterations} it={sr.iterations} "
+          f"(FP64 CG {s64.iterations}) relres={sr.rel_residual:.2e} err_vs_fp64={err:.2e}")
+    assert sr.converged and sr.rel_residual <= 1e-10
+    assert err < 1e-7
+    # FP32: about the FP64 count; BF16's operator error (2^-9 per entry) slows
+    # the recurrence on the stiffer elasticity problem (2x measured)
+    assert sr.iterations <= (1.35 if prec == PREC.FP32 else 2.5) * s64.iterations + 10
+    assert sr.precision == int(prec) | IR_RELIABLE
+    # the FP64 solution against scipy on the assembled system
+    xd = spla.spsolve(direct(p, bcop, pen), b)
+    assert rel_l2(xr.cpu().numpy(), xd) < 1e-7
