@@ -10,6 +10,9 @@
 // The host looks at the jobs' scalars once per step (one device-to-host copy);
 // jobs leave when done and new ones are admitted between steps.
 //
+// Jobs on method-2 (FREE) operators (FP64 CG) launch their in-kernel element
+// kernels one job at a time; the gather and the CG phases stay batched.
+//
 // Every job's arithmetic depends on its own data only: the scatter is the
 // deterministic transposed gather and every reduction uses the job's fixed
 // single-solve grid, so a job's result is bit-identical alone, in any mix,
@@ -90,6 +93,7 @@ struct amx_server {
     bool tables_dirty = true;
     std::map<uint64_t, Table> ebe;  // batched element launches, key: n | kt << 16 | f64 << 24
     Table gather[2], cg[2];         // [f64]
+    std::vector<Job*> free_jobs;    // jobs on method-2 (FREE) operators: element kernels launched per job
     ~amx_server()
     {
         jobs.clear();
@@ -142,13 +146,16 @@ void rebuild_tables(amx_server& S)
     std::vector<uint32_t> gp[2] = {{0}, {0}};
     std::vector<amx::CgJob> c[2];
     std::vector<uint32_t> cp[2] = {{0}, {0}};
+    S.free_jobs.clear();
     for (auto& kv : S.jobs) {
         Job& j = *kv.second;
         if (j.phase == kDone) continue;
         Op& op = j.h->op;
         const int f = j.f64iter ? 1 : 0;
         const uint32_t* skip = &S.it_sc(j)->done;
+        if (op.is_free()) S.free_jobs.push_back(&j);
         for (auto& grp : op.groups) {
+            if (op.is_free()) break;
             const uint64_t key = (uint64_t)grp.n | ((uint64_t)j.kt << 16) | ((uint64_t)f << 24);
             auto& e = ebe[key];
             if (e.second.empty()) e.second.push_back(0);
@@ -214,6 +221,11 @@ void launch_iteration(amx_server& S)
         Table& t = kv.second;
         amx::launch_ebe_batched(n, kt, f64, t.dev_jobs.as<amx::EbeJob>(), t.dev_prefix.as<uint32_t>(), t.n, t.total,
                                 s);
+    }
+    for (Job* jp : S.free_jobs) {  // FP64 element kernels from coordinates, into the job's element rows
+        Op& op = jp->h->op;
+        amx::ebe_launch(op, KType::F64, true, op.bc_operator == AMX_BCOP_IDENTITY, 2, jp->p.p, nullptr, s,
+                        &S.it_sc(*jp)->done, jp->ve.p);
     }
     for (int f = 0; f < 2; f++) {
         Table& t = S.gather[f];
@@ -424,14 +436,15 @@ int amx_server_submit(amx_server* S, amx_op* h, int method, const double* b_host
                     "inner_rtol must be in (0, 1)");
         Op& op = h->op;
         AMX_REQUIRE(op.finalized, AMX_INVALID_STATE, "operator not finalized");
-        AMX_REQUIRE(!op.is_free(), AMX_NOT_IMPLEMENTED, "served solves need stored element matrices");
+        AMX_REQUIRE(!op.is_free() || method == AMX_PREC_FP64, AMX_NOT_IMPLEMENTED,
+                    "method-2 (FREE) operators are served with FP64 CG");
         AMX_REQUIRE(op.size == 1, AMX_NOT_IMPLEMENTED, "served solves are one-rank");
         AMX_REQUIRE(op.device == S->device, AMX_INVALID_ARGUMENT, "operator on another device");
         AMX_REQUIRE(op.scatter == AMX_SCATTER_DETERMINISTIC, AMX_NOT_IMPLEMENTED,
                     "served solves use the deterministic scatter");
         AMX_REQUIRE(op.has(base), AMX_INVALID_STATE, "the method's storage was not built at finalize");
         for (auto& g : op.groups)
-            AMX_REQUIRE(amx::batched_n(g.n), AMX_NOT_IMPLEMENTED,
+            AMX_REQUIRE(op.is_free() || amx::batched_n(g.n), AMX_NOT_IMPLEMENTED,
                         "batched launches need a tile kernel for n = " + std::to_string(g.n));
         AMX_REQUIRE(!S->free_slots.empty(), AMX_REJECTED, "all job slots are in use");
         amx::DeviceGuard dg(S->device);

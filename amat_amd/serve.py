@@ -144,9 +144,12 @@ def rhs_for(p: problems.Problem, seed: int) -> np.ndarray:
     return b
 
 
-def build_operator(p: problems.Problem, precision=PREC.FP64 | PREC.FP32 | PREC.BF16) -> aMatFree:
-    op = p.build_operator(aMatFree, Maps, precision=precision, bcop=BCOP.IDENTITY)
-    return op
+def build_operator(p: problems.Problem, precision=PREC.FP64 | PREC.FP32 | PREC.BF16, free: bool = False) -> aMatFree:
+    """The served operator: stored element matrices in the given storage
+    precisions, or (free=True) method 2, K_e recomputed in-kernel (FP64 CG only)."""
+    if free:
+        return p.build_free_operator(aMatFree, Maps, bcop=BCOP.IDENTITY)
+    return p.build_operator(aMatFree, Maps, precision=precision, bcop=BCOP.IDENTITY)
 
 
 # ------------------------------------------------------------------ policy
@@ -162,18 +165,21 @@ class CachedOp:
 class OperatorCache:
     """Resident operators, LRU order; evicts unused ones to make room."""
 
-    def __init__(self, builder: Callable[[str, int], tuple] | None = None):
+    def __init__(self, builder: Callable[[str, int], tuple] | None = None, free: bool = False):
         self.items: collections.OrderedDict[tuple, CachedOp] = collections.OrderedDict()
+        self.free = free
         self.builder = builder or self._default_builder
         self.builds = 0
         self.build_seconds = 0.0
         self.evictions = 0
 
-    @staticmethod
-    def _default_builder(kind: str, n: int):
+    def _default_builder(self, kind: str, n: int):
         p = build_problem(kind, n)
-        op = build_operator(p)
+        op = build_operator(p, free=self.free)
         return p, op, int(op.info().device_bytes)
+
+    def estimate(self, kind: str, n: int) -> int:
+        return estimate_op_bytes(kind, n, free=self.free)
 
     @property
     def bytes(self) -> int:
@@ -206,18 +212,20 @@ class OperatorCache:
         return c
 
 
-def estimate_op_bytes(kind: str, n: int) -> int:
+def estimate_op_bytes(kind: str, n: int, free: bool = False) -> int:
     """Device bytes of an operator of this kind and size with FP64 + FP32 +
-    BF16 storage (element matrices, maps, deterministic-scatter tables), used
-    before the operator exists."""
+    BF16 storage (element matrices, maps, deterministic-scatter tables), or
+    with free=True the method-2 form (coordinates and a factor per element
+    instead of matrices), used before the operator exists."""
     if kind == "hex8":
-        E, nd, N = n ** 3, 8, (n + 1) ** 3
+        E, nd, N, nodes = n ** 3, 8, (n + 1) ** 3, 8
     elif kind == "hex20":
-        E, nd, N = n ** 3, 60, 3 * ((n + 1) ** 3 + 3 * n * (n + 1) ** 2)
+        E, nd, N, nodes = n ** 3, 60, 3 * ((n + 1) ** 3 + 3 * n * (n + 1) ** 2), 20
     else:
         E, nd = 6 * n ** 3, 10
-        N = (n + 1) ** 3 + 7 * n ** 3 + 3 * n * (n + 1) ** 2
-    return E * nd * nd * 14 + E * nd * (4 + 4 + 4 + 8 + 4) + N * 40
+        N, nodes = (n + 1) ** 3 + 7 * n ** 3 + 3 * n * (n + 1) ** 2, 10
+    per_elem = 24 * nodes + 8 if free else nd * nd * 14
+    return E * per_elem + E * nd * (4 + 4 + 4 + 8 + 4) + N * 40
 
 
 def estimate_job_bytes(kind: str, n: int, method: PREC) -> int:
@@ -288,7 +296,7 @@ class ServingLoop:
                 r = pending[0]
                 key = (r.kind, r.n)
                 c = self.cache.get(key)
-                need_op = 0 if c is not None else estimate_op_bytes(r.kind, r.n)
+                need_op = 0 if c is not None else self.cache.estimate(r.kind, r.n)
                 need_job = estimate_job_bytes(r.kind, r.n, self.method)
                 if not self.cache.evict_for(need_op + need_job, self.budget, workspace):
                     if not running:

@@ -92,7 +92,8 @@ def time_matvec(fn, v, u, steps, warmup, torch):
 def precision_mode(args) -> dict:
     """Config C3: jittered tet10 Poisson (n^3 hexes x 6 tets), b = M f, identity
     BCs; FP64 CG vs FP64 iterative refinement with FP32 / BF16 element
-    matrices, all to 1e-10 relative residual, error against the FP64 solution."""
+    matrices (restarted per outer step, and with reliable updates), all to
+    1e-10 relative residual, error against the FP64 solution."""
     import numpy as np
     import torch
     from amat_amd import BCOP, PREC, Maps, aMatFree, problems
@@ -131,19 +132,26 @@ def precision_mode(args) -> dict:
                       "converged": bool(st.converged)}
     log(f"[precision] FP64 CG: {st.iterations} it, {t64:.3f} s, relres {st.rel_residual:.2e}")
     xr = x64.cpu().numpy()
-    for prec in (PREC.FP32, PREC.BF16):
+    from amat_amd.amat import IR_RELIABLE
+    # restarted refinement (inner tolerance --inner-rtol) and reliable updates
+    # (replacement threshold --ru-delta)
+    for prec, reliable in ((PREC.FP32, False), (PREC.BF16, False), (PREC.FP32, True), (PREC.BF16, True)):
         x = torch.zeros_like(db)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        si = op.ir_cg(prec, db, x, rtol=1e-10, inner_rtol=args.inner_rtol, max_outer=60, max_inner=20000, stream=s)
+        si = op.ir_cg(int(prec) | (IR_RELIABLE if reliable else 0), db, x, rtol=1e-10,
+                      inner_rtol=args.ru_delta if reliable else args.inner_rtol, max_outer=200 if reliable else 60,
+                      max_inner=20000, stream=s)
         torch.cuda.synchronize()
         ti = time.perf_counter() - t
         err = float(np.linalg.norm(x.cpu().numpy() - xr) / np.linalg.norm(xr))
-        res[f"ir_cg_{prec.name.lower()}"] = {"seconds": ti, "outer": si.outer_iterations, "inner": si.iterations,
-                                             "rel_residual": si.rel_residual, "converged": bool(si.converged),
-                                             "rel_error_vs_fp64_solution": err, "speedup_vs_fp64_cg": t64 / ti}
-        log(f"[precision] IR-CG {prec.name}: outer {si.outer_iterations} inner {si.iterations}, {ti:.3f} s, "
-            f"relres {si.rel_residual:.2e}, err vs FP64 {err:.2e}, speedup {t64 / ti:.2f}x")
+        key = f"{'ru' if reliable else 'ir'}_cg_{prec.name.lower()}"
+        res[key] = {"seconds": ti, "outer": si.outer_iterations, "inner": si.iterations,
+                    "rel_residual": si.rel_residual, "converged": bool(si.converged),
+                    "rel_error_vs_fp64_solution": err, "speedup_vs_fp64_cg": t64 / ti}
+        log(f"[precision] {'RU' if reliable else 'IR'}-CG {prec.name}: {'replacements' if reliable else 'outer'} "
+            f"{si.outer_iterations} inner {si.iterations}, {ti:.3f} s, relres {si.rel_residual:.2e}, "
+            f"err vs FP64 {err:.2e}, speedup {t64 / ti:.2f}x")
     return res
 
 
@@ -313,19 +321,26 @@ def serve_mode(args) -> dict:
     rates = [float(r) for r in args.serve_rates.split(",")]
     # fp32 / bf16: refinement restarted per outer step (inner tolerance --inner-rtol);
     # fp32r / bf16r: reliable updates (replacement threshold --ru-delta)
+    # fp64free: FP64 CG on method-2 operators (K_e recomputed in-kernel; own cache)
     methods = {"fp64": PREC.FP64, "fp32": PREC.FP32, "bf16": PREC.BF16,
-               "fp32r": int(PREC.FP32) | IR_RELIABLE, "bf16r": int(PREC.BF16) | IR_RELIABLE}
+               "fp32r": int(PREC.FP32) | IR_RELIABLE, "bf16r": int(PREC.BF16) | IR_RELIABLE, "fp64free": PREC.FP64}
     names = args.serve_methods.split(",")
     streams = {r: serve.make_requests(args.serve_requests, r, seed=args.serve_seed) for r in rates}
     keys = sorted({(q.kind, q.n) for st in streams.values() for q in st})
     budget = int(args.serve_budget_gb * 2**30)
-    cache = serve.OperatorCache()
-    t0 = time.time()
-    for i, key in enumerate(keys):
-        cache.build(key)
-        if i % 10 == 0:
-            log(f"[serve] built {i + 1}/{len(keys)} operators ({cache.bytes / 2**30:.1f} GiB, {time.time() - t0:.0f}s)")
-    log(f"[serve] {len(keys)} operators resident: {cache.bytes / 2**30:.1f} GiB, built in {cache.build_seconds:.1f}s")
+    caches = {}
+    for free in sorted({n == "fp64free" for n in names}):
+        cache = serve.OperatorCache(free=free)
+        t0 = time.time()
+        for i, key in enumerate(keys):
+            cache.build(key)
+            if i % 10 == 0:
+                log(f"[serve] built {i + 1}/{len(keys)} {'method-2 ' if free else ''}operators "
+                    f"({cache.bytes / 2**30:.1f} GiB, {time.time() - t0:.0f}s)")
+        log(f"[serve] {len(keys)} {'method-2 ' if free else ''}operators resident: {cache.bytes / 2**30:.1f} GiB, "
+            f"built in {cache.build_seconds:.1f}s")
+        caches[free] = cache
+    cache = caches.get(False, caches.get(True))
     res = {"metric": "C5 served CG solves/s and latency at 1e-10 relative residual", "requests": args.serve_requests,
            "rates": rates, "budget_GiB": args.serve_budget_gb, "slots": args.serve_slots, "step_iters": args.serve_iters,
            "inner_rtol": args.inner_rtol, "ru_delta": args.ru_delta,
@@ -334,7 +349,7 @@ def serve_mode(args) -> dict:
     for rate in rates:
         for name in names:
             server = serve.Server(max_jobs=args.serve_slots)
-            loop = serve.ServingLoop(server, cache, methods[name], budget, n_iter=args.serve_iters,
+            loop = serve.ServingLoop(server, caches[name == "fp64free"], methods[name], budget, n_iter=args.serve_iters,
                                      inner_rtol=args.ru_delta if name.endswith("r") else args.inner_rtol)
             last = [0.0]
 

@@ -222,6 +222,43 @@ def test_free_two_rank_partition(torch_gpu):
         assert rel_l2(got, full[r.owned_begin:r.owned_end + 1]) <= 1e-13
 
 
+def test_free_operators_served(torch_gpu):
+    """Method-2 operators in the continuous-batching server (FP64 CG): each
+    job's solution is bit-identical to the single solve on the same operator,
+    alone and in a mix with a stored-matrix job."""
+    torch = torch_gpu
+    from amat_amd import serve
+    cases = [serve.build_problem("hex8", 6), serve.build_problem("hex20", 3), serve.build_problem("tet10", 3)]
+    ops = [serve.build_operator(p, free=True) for p in cases]
+    rhs = [serve.rhs_for(p, 40 + i) for i, p in enumerate(cases)]
+    single = []
+    for p, op, b in zip(cases, ops, rhs):
+        db = torch.from_numpy(b).cuda()
+        x = torch.zeros_like(db)
+        st = op.cg(db, x, rtol=1e-10, max_it=20000)
+        assert st.converged
+        single.append((x.cpu().numpy(), st.iterations))
+    stored_p = serve.build_problem("hex8", 5)
+    stored_op = serve.build_operator(stored_p)
+    srv = Server(max_jobs=8)
+    ids = [srv.submit(op, PREC.FP64, b) for op, b in zip(ops, rhs)]
+    sid = srv.submit(stored_op, PREC.FP64, serve.rhs_for(stored_p, 7))
+    sizes = {j: p.n_dofs for j, p in zip(ids, cases)}
+    sizes[sid] = stored_p.n_dofs
+    out = {}
+    while len(out) < len(sizes):
+        srv.step(8)
+        for j in srv.poll():
+            out[j] = srv.result(j, sizes[j])
+    for j, (x1, it1) in zip(ids, single):
+        x, st = out[j]
+        assert st.converged and st.rel_residual <= 1e-10
+        np.testing.assert_array_equal(x, x1)
+        assert st.iterations == it1
+    with pytest.raises(AmatError):
+        srv.submit(ops[0], PREC.FP32, rhs[0])
+
+
 def test_free_rejections(torch_gpu):
     torch = torch_gpu
     p = problems.poisson_hex8(3)
@@ -232,7 +269,7 @@ def test_free_rejections(torch_gpu):
     with pytest.raises(AmatError):
         MultiRHS([op], PREC.FP32)
     with pytest.raises(AmatError):
-        Server(max_jobs=2).submit(op, PREC.FP64, np.zeros(p.n_dofs))
+        Server(max_jobs=2).submit(op, PREC.FP32, np.zeros(p.n_dofs))   # FREE is served with FP64 CG only
     # wrong element kind for the DoF count; FREE finalize without coordinates
     o2 = aMatFree(p.build_maps(Maps))
     with pytest.raises(AmatError):

@@ -87,7 +87,7 @@ def test_make_requests_mix_and_rate():
 
 def test_budget_slots_fifo_and_lru(monkeypatch):
     monkeypatch.setattr(serve, "rhs_for", lambda p, seed: np.zeros(p.n_dofs))
-    monkeypatch.setattr(serve, "estimate_op_bytes", lambda kind, n: 1000 * n)
+    monkeypatch.setattr(serve, "estimate_op_bytes", lambda kind, n, free=False: 1000 * n)
     monkeypatch.setattr(serve, "estimate_job_bytes", lambda kind, n, m: 100)
     log = []
     cache = serve.OperatorCache(builder=fake_builder(log))
@@ -110,7 +110,7 @@ def test_budget_slots_fifo_and_lru(monkeypatch):
 
 def test_request_that_never_fits_raises(monkeypatch):
     monkeypatch.setattr(serve, "rhs_for", lambda p, seed: np.zeros(p.n_dofs))
-    monkeypatch.setattr(serve, "estimate_op_bytes", lambda kind, n: 10 ** 9)
+    monkeypatch.setattr(serve, "estimate_op_bytes", lambda kind, n, free=False: 10 ** 9)
     monkeypatch.setattr(serve, "estimate_job_bytes", lambda kind, n, m: 100)
     cache = serve.OperatorCache(builder=fake_builder([]))
     loop = serve.ServingLoop(FakeServer(2), cache, PREC.FP64, 1000, clock=Clock())
