@@ -525,8 +525,21 @@ __global__ __launch_bounds__(256) void mrhs_gather_kernel(const uint32_t* __rest
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (uint64_t)n_total * nrhs) return;
     const uint32_t d = (uint32_t)(i / nrhs), c = (uint32_t)(i - (uint64_t)d * nrhs);
+    const uint32_t b = off[d], cnt = off[d + 1] - b;
     float s = 0.0f;
-    for (uint32_t q = off[d], qe = off[d + 1]; q < qe; ++q) s += ve[(uint64_t)pos[q] * nrhs + c];
+    // chunks of 8 entries: index loads, then row loads, in flight together;
+    // the sum keeps the list order (entries past the end are not added)
+    for (uint32_t q0 = 0; q0 < cnt; q0 += 8) {
+        uint32_t p[8];
+        float x[8];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; j++) p[j] = pos[b + min(q0 + j, cnt - 1)];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; j++) x[j] = ve[(uint64_t)p[j] * nrhs + c];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; j++)
+            if (q0 + j < cnt) s += x[j];
+    }
     V[i] = s;
 }
 
