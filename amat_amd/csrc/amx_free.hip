@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 
 #include "amx_internal.hpp"
 #include "amx_kernels.hpp"
@@ -34,6 +35,7 @@ namespace amx {
 namespace {
 
 constexpr uint32_t kFlag = 0x80000000u, kMask = 0x7fffffffu;
+constexpr int kMaxDevices = 64;
 
 __device__ __forceinline__ void lds_sync_wave()
 {
@@ -105,8 +107,12 @@ __global__ __launch_bounds__(256) void free_laplace_kernel(const double* __restr
                 if constexpr (!DIAG) h[d] += ue[a] * dn;
             }
         double inv[3][3];
-        const double w = jac_inv_r(J, inv) * t[NN * 3] * sc;
+        const double det = jac_inv_r(J, inv);
+        double w = det * t[NN * 3] * sc;
         if constexpr (DIAG) {
+            // a non-positive Jacobian at any point marks the element (NaN diagonal);
+            // finalize rejects it, as the element routines do
+            if (!(det > 0.0)) w = __builtin_nan("");
 #pragma unroll
             for (int a = 0; a < NN; a++) {
                 double s = 0.0;
@@ -269,7 +275,9 @@ __global__ __launch_bounds__(256) void free_hex20_kernel(const double* __restric
             }
             double inv[3][3];
             const double det = jac_inv_r(J, inv);
-            const double w = e < n_here ? det * wq[q] * sc_here : 0.0;
+            double w = e < n_here ? det * wq[q] * sc_here : 0.0;
+            // DIAG: a non-positive Jacobian at a real point marks the element (NaN diagonal)
+            if (DIAG && e < n_here && q < nq && !(det > 0.0)) w = __builtin_nan("");
             double S[kH20Slot];
             if constexpr (DIAG) {
 #pragma unroll
@@ -350,17 +358,23 @@ template <int QP>
 void launch_hex20(bool masked, bool atomic, bool diag, const FreeArgs& a, hipStream_t s)
 {
     constexpr uint32_t per_block = 4u * (64u / QP);
-    static int resident = 0;  // blocks per device: CUs x blocks per CU (the LDS bounds it)
-    if (resident == 0) {
-        int dev = 0, cus = 0, per_cu = 0;
-        AMX_HIP(hipGetDevice(&dev));
+    // resident blocks per device: CUs x blocks per CU (the LDS bounds it); a
+    // benign race between threads computes the same value
+    static std::atomic<int> resident[kMaxDevices];
+    int dev = 0;
+    AMX_HIP(hipGetDevice(&dev));
+    AMX_REQUIRE(dev >= 0 && dev < kMaxDevices, AMX_NOT_IMPLEMENTED, "device ordinal beyond the supported range");
+    int res = resident[dev].load(std::memory_order_relaxed);
+    if (res == 0) {
+        int cus = 0, per_cu = 0;
         AMX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         AMX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, free_hex20_kernel<QP, false, false, false>,
                                                              256, 0));
-        resident = std::max(1, cus * std::max(per_cu, 1));
+        res = std::max(1, cus * std::max(per_cu, 1));
+        resident[dev].store(res, std::memory_order_relaxed);
     }
     const uint32_t need = (a.n_slots + per_block - 1) / per_block;
-    const dim3 grid(std::min<uint32_t>(need, (uint32_t)resident)), block(256);
+    const dim3 grid(std::min<uint32_t>(need, (uint32_t)res)), block(256);
 #define AMX_H20(M, A, D)                                                                                           \
     free_hex20_kernel<QP, M, A, D><<<grid, block, 0, s>>>(a.xe, a.scale, a.map, a.u, a.out, a.ve_base,            \
                                                           a.slot_begin, a.n_slots, a.qp, a.nq, a.c1, a.c2, a.c3,  \

@@ -272,7 +272,12 @@ void finalize_op(Op& op)
         for (uint32_t e = 0; e < op.n_elem; e++) {
             const Group& g = op.groups[op.elem_group[e]];
             const uint64_t base = g.ve_base + (uint64_t)g.elem_slot[op.elem_index[e]] * g.n;
-            for (uint32_t r = 0; r < g.n; r++) tr += hve[base + r];
+            for (uint32_t r = 0; r < g.n; r++) {
+                // the diagonal kernel marks a non-positive Jacobian with NaN
+                AMX_REQUIRE(std::isfinite(hve[base + r]), AMX_INVALID_ARGUMENT,
+                            "element " + std::to_string(e) + ": non-positive Jacobian at a quadrature point");
+                tr += hve[base + r];
+            }
         }
     } else {
         for (uint32_t e = 0; e < op.n_elem; e++) {

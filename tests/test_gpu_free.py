@@ -279,3 +279,15 @@ def test_free_rejections(torch_gpu):
         o2.finalize()
     with pytest.raises(AmatError):
         o2.set_element_function(ELEM.HEX8_LAPLACE, p.element_coords(), None, ngt=0)
+    # an inverted (mirrored) element: non-positive Jacobians, rejected at finalize
+    # as the element routines reject them
+    for kind_p, kind, perm in ((problems.poisson_hex8(3), ELEM.HEX8_LAPLACE, [1, 0, 3, 2, 5, 4, 7, 6]),
+                               (problems.elasticity_hex20(2, ngt=2), ELEM.HEX20_ELASTIC,
+                                [1, 0, 3, 2, 5, 4, 7, 6, 8, 11, 10, 9, 12, 15, 14, 13, 17, 16, 19, 18])):
+        xe = kind_p.element_coords().copy()
+        xe[1] = xe[1][perm]
+        o3 = aMatFree(kind_p.build_maps(Maps))
+        o3.set_element_function(kind, xe, None, ngt=2)
+        with pytest.raises(AmatError) as e:
+            o3.finalize()
+        assert e.value.name == "INVALID_ARGUMENT" and "element 1" in str(e.value)
