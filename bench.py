@@ -203,29 +203,34 @@ def free_mode(args) -> dict:
         res[f"ngt{ngt}_speedup"] = res[f"ngt{ngt}_stored"]["ms"] / res[f"ngt{ngt}_free"]["ms"]
         del st, fr, u, v0, v1
         torch.cuda.empty_cache()
-    # hex8 Poisson (2x2x2 Gauss), one lane per element
-    nh = 2 * args.n
-    p = problems.poisson_hex8(nh)
-    st = p.build_operator(aMatFree, Maps, bcop=BCOP.IDENTITY)
-    fr = p.build_free_operator(aMatFree, Maps, bcop=BCOP.IDENTITY)
-    u = torch.from_numpy(p.random_u()).cuda()
-    v0, v1 = torch.empty_like(u), torch.empty_like(u)
-    st.matvec_dev(v0, u, s)
-    fr.matvec_dev(v1, u, s)
-    torch.cuda.synchronize()
-    rel = float(torch.linalg.norm(v1 - v0) / torch.linalg.norm(v0))
-    for name, op in (("stored", st), ("free", fr)):
-        ms = time_matvec(op.matmult_dev, v0, u, args.steps, args.warmup, torch)
-        ms_e, ms_g, _ = op.time_kernels_dev(args.steps, v0, u, s)
-        inf = op.info()
-        res[f"hex8_{nh}_{name}"] = {"ms": ms, "GDoF/s": p.n_dofs / ms / 1e6, "element_kernel_ms": ms_e,
-                                    "gather_ms": ms_g, "device_bytes": inf.device_bytes,
-                                    "alg_bytes_per_matvec": inf.alg_bytes_fp64,
-                                    "hbm_frac": inf.alg_bytes_fp64 / (ms * 1e-3) / HBM_PEAK}
-        if name == "free":
-            res[f"hex8_{nh}_free"]["rel_l2_vs_stored"] = rel
-        log(f"[free] hex8 {nh}^3 {name}: {ms:.4f} ms/matvec ({p.n_dofs / ms / 1e6:.2f} GDoF/s), element kernel "
-            f"{ms_e:.4f} ms, gather {ms_g:.4f} ms")
+    # Laplace kinds, one lane per element: hex8 Poisson (2x2x2 Gauss) at (2n)^3,
+    # jittered tet10 Poisson (8-point rule) at n^3 x 6
+    for label, p in ((f"hex8_{2 * args.n}", problems.poisson_hex8(2 * args.n)),
+                     (f"tet10_{args.n}", problems.poisson_tet10(args.n, with_rhs=False))):
+        st = p.build_operator(aMatFree, Maps, bcop=BCOP.IDENTITY)
+        fr = p.build_free_operator(aMatFree, Maps, bcop=BCOP.IDENTITY)
+        u = torch.from_numpy(p.random_u()).cuda()
+        v0, v1 = torch.empty_like(u), torch.empty_like(u)
+        st.matvec_dev(v0, u, s)
+        fr.matvec_dev(v1, u, s)
+        torch.cuda.synchronize()
+        rel = float(torch.linalg.norm(v1 - v0) / torch.linalg.norm(v0))
+        for name, op in (("stored", st), ("free", fr)):
+            ms = time_matvec(op.matmult_dev, v0, u, args.steps, args.warmup, torch)
+            ms_e, ms_g, _ = op.time_kernels_dev(args.steps, v0, u, s)
+            inf = op.info()
+            r = {"ms": ms, "GDoF/s": p.n_dofs / ms / 1e6, "element_kernel_ms": ms_e, "gather_ms": ms_g,
+                 "device_bytes": inf.device_bytes, "alg_bytes_per_matvec": inf.alg_bytes_fp64,
+                 "hbm_frac": inf.alg_bytes_fp64 / (ms * 1e-3) / HBM_PEAK}
+            if name == "free":
+                r["rel_l2_vs_stored"] = rel
+                r["element_kernel_TFLOP/s"] = inf.flops / (ms_e * 1e-3) / 1e12
+            res[f"{label}_{name}"] = r
+            log(f"[free] {label} {name}: {ms:.4f} ms/matvec ({p.n_dofs / ms / 1e6:.2f} GDoF/s), element kernel "
+                f"{ms_e:.4f} ms, gather {ms_g:.4f} ms" +
+                (f", {r['element_kernel_TFLOP/s']:.2f} TFLOP/s, rel L2 vs stored {rel:.2e}" if name == "free" else ""))
+        del st, fr, u, v0, v1, p
+        torch.cuda.empty_cache()
     return res
 
 
