@@ -83,6 +83,7 @@ SIGNATURES = {
     "amx_op_set_element_matrices": (I32, [P, U32, U32, P, I32]),
     "amx_op_set_element_matrices_scaled": (I32, [P, P, I32, P]),
     "amx_op_set_element_function": (I32, [P, I32, P, P, F64, F64, U32]),
+    "amx_measure_read_bandwidth": (I32, [I32, U64, I32, C.POINTER(F64)]),
     "amx_op_set_options": (I32, [P, I32, I32]),
     "amx_op_set_penalty": (I32, [P, F64]),
     "amx_op_set_bc_operator": (I32, [P, I32]),

@@ -264,6 +264,13 @@ class Maps:
     def get_RecvRankIds(self): return np.nonzero(self._recv()[0])[0].astype(np.uint32)
 
 
+def measure_read_bandwidth(device: int = 0, nbytes: int = 1 << 30, iters: int = 10) -> float:
+    """Achievable HBM read bandwidth of the device in bytes/s (amx_measure_read_bandwidth)."""
+    v = C.c_double()
+    call("amx_measure_read_bandwidth", int(device), int(nbytes), int(iters), C.byref(v))
+    return v.value
+
+
 class aMatFree:
     """par::aMatFree (include/aMatFree.hpp) running MATFREE_TYPE 2 (built-in element
     kinds) and 3/4/5 on one GPU."""

@@ -15,6 +15,7 @@
 // tiles / DoFs: its L2 then holds one slab of u / ve instead of all of it.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <type_traits>
 
@@ -753,3 +754,68 @@ void launch_f2d(const float* a, double* b, uint64_t n, hipStream_t s)
 }
 
 }  // namespace amx
+
+// ---------------------------------------------------------- measurement aid
+namespace amx {
+namespace {
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// read-only stream: 16-byte non-temporal loads, grid-stride; the xor keeps
+// the loads live and the (practically never taken) store keeps the xor
+__global__ __launch_bounds__(256) void read_stream_kernel(const u32x4* __restrict__ p, uint64_t n16,
+                                                         uint32_t* __restrict__ out)
+{
+    uint32_t acc = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * 256u) {
+        const u32x4 v = __builtin_nontemporal_load(p + i);
+        acc ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    if (acc == 0x9e3779b9u) out[blockIdx.x] = acc;
+}
+
+}  // namespace
+}  // namespace amx
+
+extern "C" int amx_measure_read_bandwidth(int device, uint64_t bytes, int iters, double* bytes_per_s)
+{
+    return amx::guarded(__func__, [&] {
+        AMX_REQUIRE(bytes_per_s && iters > 0 && bytes >= (1u << 20), AMX_INVALID_ARGUMENT,
+                    "need bytes >= 1 MiB, iters > 0 and an output");
+        int ndev = 0;
+        AMX_HIP(hipGetDeviceCount(&ndev));
+        AMX_REQUIRE(device >= 0 && device < ndev, AMX_INVALID_ARGUMENT, "device not present");
+        int prev = -1;
+        AMX_HIP(hipGetDevice(&prev));
+        AMX_HIP(hipSetDevice(device));
+        struct Restore {
+            int d;
+            ~Restore() { if (d >= 0) (void)hipSetDevice(d); }
+        } restore{prev};
+        const uint64_t n16 = bytes / 16;
+        amx::DevBuf buf, out;
+        buf.alloc(n16 * 16);
+        int cus = 0;
+        AMX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+        const uint32_t grid = (uint32_t)std::max(1, cus) * 8u;
+        out.alloc((size_t)grid * sizeof(uint32_t));
+        AMX_HIP(hipMemset(buf.p, 0, buf.bytes));
+        hipStream_t s = nullptr;
+        AMX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        AMX_HIP(hipEventCreate(&e0));
+        AMX_HIP(hipEventCreate(&e1));
+        amx::read_stream_kernel<<<grid, 256, 0, s>>>(buf.as<amx::u32x4>(), n16, out.as<uint32_t>());  // warm-up
+        AMX_HIP(hipEventRecord(e0, s));
+        for (int i = 0; i < iters; i++)
+            amx::read_stream_kernel<<<grid, 256, 0, s>>>(buf.as<amx::u32x4>(), n16, out.as<uint32_t>());
+        AMX_HIP(hipEventRecord(e1, s));
+        AMX_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        AMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
+        *bytes_per_s = (double)n16 * 16.0 * iters / (ms * 1e-3);
+    });
+}

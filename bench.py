@@ -48,19 +48,38 @@ def cpu_baseline(p, n_sample_layers: int, seconds: float, threads: int) -> dict:
     gm = p.gmap[:ns]
     K = p.element_matrices(0, ns)
     u = p.random_u()
-    t0 = time.perf_counter()
-    orc.matvec(gm, K, u, n_total=p.n_dofs, nthreads=threads)  # warm-up, and the per-matvec estimate
-    t1 = time.perf_counter() - t0
-    reps = max(1, int(seconds / max(t1, 1e-6)))
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        orc.matvec(gm, K, u, n_total=p.n_dofs, nthreads=threads)
-    dt = (time.perf_counter() - t0) / reps
+
+    def timed(layout):
+        t0 = time.perf_counter()
+        orc.matvec(gm, K, u, n_total=p.n_dofs, nthreads=threads, layout=layout)  # warm-up and estimate
+        t1 = time.perf_counter() - t0
+        reps = max(1, int(seconds / max(t1, 1e-6)))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            orc.matvec(gm, K, u, n_total=p.n_dofs, nthreads=threads, layout=layout)
+        return (time.perf_counter() - t0) / reps, reps
+
+    dt, reps = timed(0)
+    # the VECTORIZED_OPENMP variant (column AXPY with omp simd): K_e is symmetric, so
+    # its column-major storage is the same array
+    dt1, reps1 = timed(1)
     t_full = dt * E / ns
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": p.n_dofs / t_full / 1e9, "unit": "GDoF/s", "cores": threads, "kind": "port",
             "sample": f"{ns} of {E} elements ({ns // per_layer} element layers), {reps} matvecs "
                       f"({dt * reps:.1f} s), {dt * 1e3:.1f} ms each, scaled by element count; oracle "
-                      f"matvec_ghosted_OMP (row-major scalar GEMV + omp atomic), {threads} OpenMP threads"}
+                      f"matvec_ghosted_OMP (row-major scalar GEMV + omp atomic), {threads} OpenMP threads",
+            "cpu_model": model,
+            "vectorized_openmp_value": p.n_dofs / (dt1 * E / ns) / 1e9,
+            "vectorized_openmp_sample": f"{reps1} matvecs, {dt1 * 1e3:.1f} ms each (column AXPY + omp simd, "
+                                        f"the oracle built -O2 without -march)"}
 
 
 def load_traffic(name: str):
@@ -467,6 +486,10 @@ def main() -> int:
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    # measured peak beside the nominal one: a 16-byte streaming read of 2 GiB
+    from amat_amd.amat import measure_read_bandwidth
+    read_bw = measure_read_bandwidth(torch.cuda.current_device(), 2 << 30, 10)
+    log(f"[bench] measured HBM read stream: {read_bw / 1e9:.0f} GB/s")
     E, n, N = p.n_elem, p.dofs_per_elem, p.n_dofs
     elem_bytes = E * (n * n * 8 + 4 * n) + N * 8       # K + map + u read
     alg_bytes = info.alg_bytes_fp64                     # + v write: the whole matvec
@@ -491,6 +514,7 @@ def main() -> int:
         "matvec_hbm_frac": alg_bytes / (ms * 1e-3) / HBM_PEAK,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": "ebe_tile_kernel<60,double>",
+                     "measured_read_GBps": read_bw / 1e9, "frac_of_measured_read": achieved / read_bw,
                      "alg_bytes_per_launch": elem_bytes, "ms_per_launch": ms_elem, "gather_ms": ms_gather},
     }
     if not args.no_cpu:

@@ -119,6 +119,11 @@ const char* amx_version(void);
 /* Number of visible HIP devices (0 when none); never fails. */
 int amx_device_count(void);
 
+/* Measurement aid, no counterpart in the reference: the achievable HBM read
+ * bandwidth of `device` (16-byte streaming loads over a `bytes` buffer, `iters`
+ * timed passes, hipEvents), the measured peak beside the nominal one. */
+int amx_measure_read_bandwidth(int device, uint64_t bytes, int iters, double* bytes_per_s);
+
 /* ------------------------------------------------------------------ maps
  * par::Maps<DT,GI,LI> (include/maps.hpp).  One handle per rank.  In the
  * reference the counts of all ranks come from MPI_Allgather (maps.hpp:547) and
