@@ -4,7 +4,7 @@ oracle on the same seeded inputs.
 Bar: the deterministic FP64 path is bit-exact against the oracle's
 sequential matvec_ghosted_OMP / MatMult_mf (same products, same summation
 order, no FMA contraction); the atomic-scatter and penalty paths agree to
-1e-13 relative L2; FP32/BF16 storage errors are
+1e-13 relative L2 (the stated FP64 tolerance); FP32/BF16 storage errors are
 bounded and reported.
 """
 import numpy as np

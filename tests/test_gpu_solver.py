@@ -86,7 +86,7 @@ def test_ir_cg(torch_gpu, prec, which):
 def test_reliable_update_cg(torch_gpu, prec, which):
     """Mixed-precision CG with reliable updates: the low-precision recurrence is
     never restarted, so it takes about the FP64 iteration count; convergence
-    is judged on the FP64 true residual."""
+    is decided on the FP64 true residual."""
     torch = torch_gpu
     from amat_amd.amat import IR_RELIABLE
     if which == "hex8":
