@@ -112,7 +112,7 @@ uint64_t job_bytes(const Op& op, int method)
 {
     const uint64_t n = std::max<uint32_t>(op.n_total, 1);
     const uint64_t ve = std::max<uint64_t>(op.ve_rows, 1) * 8;
-    const uint64_t partials = 3 * 1024 * 8;
+    const uint64_t partials = amx::kPartialsBytes;
     if (method == AMX_PREC_FP64) return 7 * n * 8 + ve + partials;           // b x dinv r z p q
     if (method & AMX_IR_RELIABLE) return 5 * n * 8 + 5 * n * 4 + ve + partials;  // b x dinv r64 q64 + r z p q xs
     return 5 * n * 8 + 6 * n * 4 + ve + partials;                           // b x dinv r64 q64 + r z p q xi b32
@@ -468,7 +468,8 @@ int amx_server_submit(amx_server* S, amx_op* h, int method, const double* b_host
         j.x.alloc(n * 8);
         j.dinv.alloc(n * 8);
         j.ve.alloc(std::max<uint64_t>(op.ve_rows, 1) * 8);
-        j.partials.alloc(3 * 1024 * 8);
+        j.partials.alloc(amx::kPartialsBytes);
+        AMX_HIP(hipMemsetAsync(j.partials.p, 0, amx::kPartialsBytes, s));  // reduction counters start at zero
         for (DevBuf* v : {&j.r, &j.z, &j.p, &j.q}) v->alloc(n * vs);
         if (!j.f64iter) {
             j.xi_buf.alloc(n * 4);

@@ -45,7 +45,10 @@ void ensure_ws(Op& op)
     const size_t n = std::max<uint32_t>(op.n_total, 1);
     for (DevBuf* b : {&op.ws_r, &op.ws_z, &op.ws_p, &op.ws_q, &op.ws_dinv}) ensure(*b, n * sizeof(double));
     for (DevBuf* b : {&op.ws_x32, &op.ws_b32, &op.ws_r32, &op.ws_z32, &op.ws_p32, &op.ws_q32}) ensure(*b, n * sizeof(float));
-    ensure(op.ws_partials, 3 * 1024 * sizeof(double));
+    if (op.ws_partials.bytes < kPartialsBytes) {
+        op.ws_partials.alloc(kPartialsBytes);
+        AMX_HIP(hipMemset(op.ws_partials.p, 0, kPartialsBytes));  // reduction counters start at zero
+    }
     ensure(op.ws_sc, 2 * sizeof(CgScalars));
     if (!op.host_sc) {
         void* p = nullptr;

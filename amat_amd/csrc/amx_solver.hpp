@@ -35,6 +35,10 @@ struct CgJob {
     uint32_t n, nblk;  // nblk = reduction_grid(n)
 };
 
+// Reductions: at most kMaxRedBlocks blocks; their partial sums, group sums and
+// counters live in a per-solve buffer of kPartialsBytes, zeroed when allocated.
+constexpr uint32_t kMaxRedBlocks = 1024;
+constexpr size_t kPartialsBytes = 4096 * sizeof(double);
 uint32_t reduction_grid(uint32_t n);
 void cg_init(bool f64, const void* b, const double* dinv, void* x, void* r, void* z, void* p, uint32_t n,
              double* partials, CgScalars* sc, hipStream_t s);

@@ -2,8 +2,8 @@
 //
 // All inner products accumulate in FP64 whatever the vector type.  Reductions
 // are deterministic: a fixed grid, a fixed element-to-thread assignment, an
-// LDS tree per block, and the partials summed in block order by whichever
-// block finishes last (device-scope fences around a counter).  The scalars
+// LDS tree per block, and the partials summed in a fixed tree order by the
+// blocks that finish last (per group, then overall).  The scalars
 // (alpha, beta, residual norms, the converged flag) never leave the device;
 // every kernel becomes a no-op once `done` is set.
 #include <hip/hip_runtime.h>
@@ -16,6 +16,11 @@ namespace amx {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr uint32_t kGroup = 32;  // blocks per first-level reduction group (kMaxRedBlocks / kGroup <= kBlock)
+static_assert(kMaxRedBlocks / kGroup <= (uint32_t)kBlock && kGroup <= (uint32_t)kBlock, "reduction layout");
+constexpr size_t kGroupOff = 3 * kMaxRedBlocks;        // doubles: group sums [NV][kGroup]
+constexpr size_t kCntOff = kGroupOff + 3 * kGroup;     // doubles: group counters (uint32) from here
+static_assert((kCntOff + kMaxRedBlocks / kGroup) * sizeof(double) <= kPartialsBytes, "partials buffer size");
 
 __device__ __forceinline__ double block_sum(double x, double* sh)
 {
@@ -38,16 +43,29 @@ __device__ __forceinline__ double block_sum(double x, double* sh)
 // writes back / invalidates the whole L2 of the XCD (buffer_wbl2 / buffer_inv),
 // which, once per block, dominated the solver kernels.  Instead the partials
 // are deposited with agent-scope atomic exchanges (performed at the point of
-// coherence of all XCDs) whose completion is awaited before the counter is
+// coherence of all XCDs) whose completion is awaited before a counter is
 // incremented, and the finishing block reads them with agent-scope atomic
-// loads.  Summation order is unchanged (fixed, by block index).
+// loads.
+//
+// Two levels of counters: atomics on one address serialize at the point of
+// coherence, so the blocks count per group of kGroup (the group's last block
+// sums its group and deposits the group sum), and only the group finishers
+// count on the top counter.  The summation order is fixed (a tree over the
+// blocks of a group, then a tree over the groups).  Group counters and group
+// sums live after the block partials (kGroupOff, kCntOff); the counters start
+// at zero (the buffer is cleared when allocated) and each finisher resets its
+// own.
 template <int NV>
-__device__ bool reduce_partials_n(const double (&v)[NV], double* partials, uint32_t* counter, double (&tot)[NV],
+__device__ bool reduce_partials_n(const double (&v)[NV], double* partials, uint32_t* top, double (&tot)[NV],
                                   double* sh, uint32_t blk, uint32_t nblk)
 {
-    __shared__ bool last;
+    __shared__ uint32_t stage;  // 0: done here, 1: finishes its group, 2: finishes the reduction
     double b[NV];
     for (int k = 0; k < NV; k++) b[k] = block_sum(v[k], sh);
+    const uint32_t n_groups = (nblk + kGroup - 1) / kGroup, g = blk / kGroup;
+    const uint32_t g0 = g * kGroup, gsize = min(kGroup, nblk - g0);
+    double* gpart = partials + kGroupOff;
+    uint32_t* gcount = reinterpret_cast<uint32_t*>(partials + kCntOff);
     if (threadIdx.x == 0) {
         double old = 0.0;
         for (int k = 0; k < NV; k++)
@@ -56,18 +74,41 @@ __device__ bool reduce_partials_n(const double (&v)[NV], double* partials, uint3
         // every exchange has returned (its value is used) before the counter moves
         const uint32_t one = old == old ? 1u : 1u;
         __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        const uint32_t prev = __hip_atomic_fetch_add(counter, one, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        last = (prev == nblk - 1);
+        const uint32_t prev = __hip_atomic_fetch_add(&gcount[g], one, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        stage = (prev == gsize - 1) ? 1u : 0u;
     }
     __syncthreads();
-    if (!last) return false;
+    if (stage == 0) return false;
+    // the group's finisher: its blocks' partials -> the group sum
+    double gs[NV];
     for (int k = 0; k < NV; k++) {
-        double s = 0.0;
-        for (uint32_t i = threadIdx.x; i < nblk; i += kBlock)
-            s += __hip_atomic_load(&partials[(size_t)k * nblk + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        tot[k] = block_sum(s, sh);
+        const double x = threadIdx.x < gsize
+                             ? __hip_atomic_load(&partials[(size_t)k * nblk + g0 + threadIdx.x], __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT)
+                             : 0.0;
+        gs[k] = block_sum(x, sh);
     }
-    if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) {
+        double old = 0.0;
+        for (int k = 0; k < NV; k++)
+            old += __hip_atomic_exchange(&gpart[(size_t)k * kGroup + g], gs[k], __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&gcount[g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t one = old == old ? 1u : 1u;
+        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        const uint32_t prev = __hip_atomic_fetch_add(top, one, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        stage = (prev == n_groups - 1) ? 2u : 0u;
+    }
+    __syncthreads();
+    if (stage != 2) return false;
+    for (int k = 0; k < NV; k++) {
+        const double x = threadIdx.x < n_groups
+                             ? __hip_atomic_load(&gpart[(size_t)k * kGroup + threadIdx.x], __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT)
+                             : 0.0;
+        tot[k] = block_sum(x, sh);
+    }
+    if (threadIdx.x == 0) __hip_atomic_store(top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return true;
 }
 
@@ -418,7 +459,7 @@ __global__ void cg_reset_kernel(CgScalars* sc, double tol2, uint32_t max_it, dou
 uint32_t reduction_grid(uint32_t n)
 {
     const uint32_t want = (n + kBlock * 4 - 1) / (kBlock * 4);
-    return want < 1 ? 1 : (want > 1024 ? 1024 : want);
+    return want < 1 ? 1 : (want > kMaxRedBlocks ? kMaxRedBlocks : want);
 }
 
 void cg_init(bool f64, const void* b, const double* dinv, void* x, void* r, void* z, void* p, uint32_t n,

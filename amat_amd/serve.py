@@ -237,7 +237,7 @@ def estimate_job_bytes(kind: str, n: int, method: PREC) -> int:
         E, nd = 6 * n ** 3, 10
         N = (n + 1) ** 3 + 7 * n ** 3 + 3 * n * (n + 1) ** 2
     ve = E * nd * 8
-    return (7 * N * 8 if int(method) == PREC.FP64 else 5 * N * 8 + 6 * N * 4) + ve + 3 * 1024 * 8
+    return (7 * N * 8 if int(method) == PREC.FP64 else 5 * N * 8 + 6 * N * 4) + ve + 4096 * 8
 
 
 @dataclasses.dataclass
