@@ -71,9 +71,10 @@ __device__ __forceinline__ double jac_inv_r(const double J[3][3], double inv[3][
 template <int NN, bool MASKED, bool ATOMIC, bool DIAG>
 __global__ __launch_bounds__(256) void free_laplace_kernel(const double* __restrict__ xe, const double* __restrict__ scale,
                                                           const uint32_t* __restrict__ map, const double* __restrict__ u,
-                                                          double* __restrict__ out, uint64_t ve_base, uint32_t slot_begin,
-                                                          uint32_t n_slots, uint32_t stride, const double* __restrict__ qp,
-                                                          uint32_t nq, const uint32_t* __restrict__ skip)
+                                                          double* __restrict__ out, uint64_t ve_base, uint32_t rs,
+                                                          uint32_t ss, uint32_t slot_begin, uint32_t n_slots,
+                                                          uint32_t stride, const double* __restrict__ qp, uint32_t nq,
+                                                          const uint32_t* __restrict__ skip)
 {
     if (skip && *skip) return;
     const uint32_t sl = blockIdx.x * blockDim.x + threadIdx.x;
@@ -136,7 +137,7 @@ __global__ __launch_bounds__(256) void free_laplace_kernel(const double* __restr
 #pragma unroll
     for (int a = 0; a < NN; a++) {
         if constexpr (ATOMIC) unsafeAtomicAdd(&out[m[a] & kMask], f[a]);
-        else out[ve_base + slot * NN + a] = f[a];
+        else out[ve_base + (uint64_t)a * rs + slot * ss] = f[a];
     }
 }
 
@@ -164,8 +165,9 @@ template <int QP> constexpr int h20_wave_doubles()
 template <int QP, bool MASKED, bool ATOMIC, bool DIAG>
 __global__ __launch_bounds__(256) void free_hex20_kernel(const double* __restrict__ xe, const double* __restrict__ scale,
                                                         const uint32_t* __restrict__ map, const double* __restrict__ u,
-                                                        double* __restrict__ out, uint64_t ve_base, uint32_t slot_begin,
-                                                        uint32_t n_slots, const double* __restrict__ qp, uint32_t nq,
+                                                        double* __restrict__ out, uint64_t ve_base, uint32_t rs,
+                                                        uint32_t ss, uint32_t slot_begin, uint32_t n_slots,
+                                                        const double* __restrict__ qp, uint32_t nq,
                                                         double c1, double c2, double c3,
                                                         const uint32_t* __restrict__ skip)
 {
@@ -346,7 +348,7 @@ __global__ __launch_bounds__(256) void free_hex20_kernel(const double* __restric
                 if (e < n_here) {
                     const uint64_t slot = slot0 + e;
                     if constexpr (ATOMIC) unsafeAtomicAdd(&out[map[slot * 60u + lane] & kMask], f[e]);
-                    else out[ve_base + slot * 60u + lane] = f[e];
+                    else out[ve_base + (uint64_t)lane * rs + slot * ss] = f[e];
                 }
             }
         }
@@ -376,9 +378,9 @@ void launch_hex20(bool masked, bool atomic, bool diag, const FreeArgs& a, hipStr
     const uint32_t need = (a.n_slots + per_block - 1) / per_block;
     const dim3 grid(std::min<uint32_t>(need, (uint32_t)res)), block(256);
 #define AMX_H20(M, A, D)                                                                                           \
-    free_hex20_kernel<QP, M, A, D><<<grid, block, 0, s>>>(a.xe, a.scale, a.map, a.u, a.out, a.ve_base,            \
-                                                          a.slot_begin, a.n_slots, a.qp, a.nq, a.c1, a.c2, a.c3,  \
-                                                          a.skip)
+    free_hex20_kernel<QP, M, A, D><<<grid, block, 0, s>>>(a.xe, a.scale, a.map, a.u, a.out, a.ve_base, a.ve_rs,  \
+                                                          a.ve_ss, a.slot_begin, a.n_slots, a.qp, a.nq, a.c1, a.c2, \
+                                                          a.c3, a.skip)
     if (diag) AMX_H20(false, false, true);
     else if (masked) { if (atomic) AMX_H20(true, true, false); else AMX_H20(true, false, false); }
     else { if (atomic) AMX_H20(false, true, false); else AMX_H20(false, false, false); }
@@ -390,8 +392,9 @@ void launch_laplace(bool masked, bool atomic, bool diag, const FreeArgs& a, hipS
 {
     const dim3 grid((a.n_slots + 255) / 256), block(256);
 #define AMX_LAP(M, A, D)                                                                                           \
-    free_laplace_kernel<NN, M, A, D><<<grid, block, 0, s>>>(a.xe, a.scale, a.map, a.u, a.out, a.ve_base,          \
-                                                            a.slot_begin, a.n_slots, a.stride, a.qp, a.nq, a.skip)
+    free_laplace_kernel<NN, M, A, D><<<grid, block, 0, s>>>(a.xe, a.scale, a.map, a.u, a.out, a.ve_base, a.ve_rs, \
+                                                            a.ve_ss, a.slot_begin, a.n_slots, a.stride, a.qp, a.nq, \
+                                                            a.skip)
     if (diag) AMX_LAP(false, false, true);
     else if (masked) { if (atomic) AMX_LAP(true, true, false); else AMX_LAP(true, false, false); }
     else { if (atomic) AMX_LAP(false, true, false); else AMX_LAP(false, false, false); }

@@ -154,10 +154,13 @@ template <typename VT> __device__ __forceinline__ void atomic_add(VT* p, VT x) {
 
 // ---------------------------------------------------------------- EbE tile
 // One wave, tile t: G elements x N rows (lane = g*N + r).
+// Element row (slot, r) of the deterministic scatter is ve[ve_base + r*rs + slot*ss]:
+// rs = 1, ss = N (rows of an element together) or rs = n_slots, ss = 1 (row r of
+// consecutive elements together: what a wave of consecutive DoFs reads in the gather).
 template <int N, typename KT, typename VT, bool MASKED, bool ATOMIC>
 __device__ __forceinline__ void ebe_tile_body(const KT* __restrict__ K, const uint32_t* __restrict__ map,
                                               const VT* __restrict__ u, VT* __restrict__ out, uint64_t ve_base,
-                                              uint32_t t, const float* __restrict__ kdiag)
+                                              uint32_t rs, uint32_t ss, uint32_t t, const float* __restrict__ kdiag)
 {
     using AT = VT;
     constexpr int G = 64 / N;
@@ -186,21 +189,26 @@ __device__ __forceinline__ void ebe_tile_body(const KT* __restrict__ K, const ui
     // off-diagonal rounding (row sums preserved); the BF16 diagonal slot is 0
     if constexpr (std::is_same<KT, uint16_t>::value) acc = madd(acc, (AT)kdiag[row], ue);
     if (lane < (uint32_t)GN) {
-        if constexpr (ATOMIC) atomic_add(&out[m & kIdxMask], acc);
-        else out[ve_base + row] = acc;
+        if constexpr (ATOMIC) {
+            atomic_add(&out[m & kIdxMask], acc);
+        } else {
+            const uint64_t slot = (uint64_t)t * G + g, r = lc - g * N;
+            out[ve_base + r * rs + slot * ss] = acc;
+        }
     }
 }
 
 template <int N, typename KT, typename VT, bool MASKED, bool ATOMIC>
 __global__ __launch_bounds__(256) void ebe_tile_kernel(const KT* __restrict__ K, const uint32_t* __restrict__ map,
                                                       const VT* __restrict__ u, VT* __restrict__ out,
-                                                      uint64_t ve_base, uint32_t tile_begin, uint32_t n_tiles,
-                                                      const uint32_t* __restrict__ skip, const float* __restrict__ kdiag)
+                                                      uint64_t ve_base, uint32_t rs, uint32_t ss, uint32_t tile_begin,
+                                                      uint32_t n_tiles, const uint32_t* __restrict__ skip,
+                                                      const float* __restrict__ kdiag)
 {
     if (skip && *skip) return;  // converged solver: nothing to do
     const uint32_t tloc = xcd_block(blockIdx.x, gridDim.x) * 4u + (threadIdx.x >> 6);
     if (tloc >= n_tiles) return;  // uniform per wave
-    ebe_tile_body<N, KT, VT, MASKED, ATOMIC>(K, map, u, out, ve_base, tile_begin + tloc, kdiag);
+    ebe_tile_body<N, KT, VT, MASKED, ATOMIC>(K, map, u, out, ve_base, rs, ss, tile_begin + tloc, kdiag);
 }
 
 // index of the entry that owns work item x: the largest j < n with prefix[j] <= x
@@ -230,10 +238,11 @@ __global__ __launch_bounds__(256) void ebe_tile_batched_kernel(const EbeJob* __r
     const uint32_t t = jb.tile_begin + (gt - prefix[j]);
     if (jb.masked)
         ebe_tile_body<N, KT, VT, true, false>(static_cast<const KT*>(jb.K), jb.map, static_cast<const VT*>(jb.u),
-                                              static_cast<VT*>(jb.out), jb.ve_base, t, jb.kdiag);
+                                              static_cast<VT*>(jb.out), jb.ve_base, jb.ve_rs, jb.ve_ss, t, jb.kdiag);
     else
         ebe_tile_body<N, KT, VT, false, false>(static_cast<const KT*>(jb.K), jb.map, static_cast<const VT*>(jb.u),
-                                               static_cast<VT*>(jb.out), jb.ve_base, t, jb.kdiag);
+                                               static_cast<VT*>(jb.out), jb.ve_base, jb.ve_rs, jb.ve_ss, t,
+                                               jb.kdiag);
 }
 
 // Any n: one wave per element, column-major storage (G = 1, P = 1), element
@@ -243,8 +252,9 @@ constexpr int kGenericMaxN = 1024;
 template <typename KT, typename VT, bool MASKED, bool ATOMIC>
 __global__ __launch_bounds__(256) void ebe_generic_kernel(const KT* __restrict__ K, const uint32_t* __restrict__ map,
                                                          const VT* __restrict__ u, VT* __restrict__ out,
-                                                         uint64_t ve_base, uint32_t n, uint32_t slot_begin,
-                                                         uint32_t n_slots, const uint32_t* __restrict__ skip,
+                                                         uint64_t ve_base, uint32_t rs, uint32_t ss, uint32_t n,
+                                                         uint32_t slot_begin, uint32_t n_slots,
+                                                         const uint32_t* __restrict__ skip,
                                                          const float* __restrict__ kdiag)
 {
     if (skip && *skip) return;
@@ -271,7 +281,7 @@ __global__ __launch_bounds__(256) void ebe_generic_kernel(const KT* __restrict__
         }
         if constexpr (std::is_same<KT, uint16_t>::value) acc = madd(acc, (VT)kdiag[slot * n + r], ue[w][r]);
         if constexpr (ATOMIC) atomic_add(&out[map[slot * n + r] & kIdxMask], acc);
-        else out[ve_base + slot * n + r] = acc;
+        else out[ve_base + (uint64_t)r * rs + slot * ss] = acc;
     }
 }
 
@@ -425,7 +435,8 @@ __global__ void bf16_diag_kernel(const double* __restrict__ src, const double* _
     out[i] = (float)d;
 }
 
-__global__ void extract_diag_kernel(const double* __restrict__ K, TiledIndex ti, uint64_t n_rows, double* __restrict__ ve)
+__global__ void extract_diag_kernel(const double* __restrict__ K, TiledIndex ti, uint64_t n_rows, uint32_t rs,
+                                    uint32_t ss, double* __restrict__ ve)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;  // slot * N + r
     if (i >= n_rows) return;
@@ -435,7 +446,7 @@ __global__ void extract_diag_kernel(const double* __restrict__ K, TiledIndex ti,
     const uint32_t g = (uint32_t)(slot - t * ti.G);
     const uint32_t GN = ti.G * ti.N, NP = ti.N / ti.P;
     const uint64_t k = ((t * NP + r / ti.P) * GN + g * ti.N + r) * ti.P + r % ti.P;
-    ve[i] = K[k];
+    ve[(uint64_t)r * rs + slot * ss] = K[k];
 }
 
 template <typename VT> __global__ void fill_kernel(VT* __restrict__ p, uint64_t n, VT value)
@@ -491,11 +502,11 @@ void launch_tile(bool masked, bool atomic, const EbeArgs& a, hipStream_t s)
     const VT* u = static_cast<const VT*>(a.u);
     VT* out = static_cast<VT*>(a.out);
     if (masked) {
-        if (atomic) ebe_tile_kernel<N, KT, VT, true, true><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
-        else ebe_tile_kernel<N, KT, VT, true, false><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
+        if (atomic) ebe_tile_kernel<N, KT, VT, true, true><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.ve_rs, a.ve_ss, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
+        else ebe_tile_kernel<N, KT, VT, true, false><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.ve_rs, a.ve_ss, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
     } else {
-        if (atomic) ebe_tile_kernel<N, KT, VT, false, true><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
-        else ebe_tile_kernel<N, KT, VT, false, false><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
+        if (atomic) ebe_tile_kernel<N, KT, VT, false, true><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.ve_rs, a.ve_ss, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
+        else ebe_tile_kernel<N, KT, VT, false, false><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.ve_rs, a.ve_ss, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
     }
 }
 
@@ -507,11 +518,11 @@ void launch_generic(uint32_t n, bool masked, bool atomic, const EbeArgs& a, hipS
     const VT* u = static_cast<const VT*>(a.u);
     VT* out = static_cast<VT*>(a.out);
     if (masked) {
-        if (atomic) ebe_generic_kernel<KT, VT, true, true><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, n, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
-        else ebe_generic_kernel<KT, VT, true, false><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, n, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
+        if (atomic) ebe_generic_kernel<KT, VT, true, true><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.ve_rs, a.ve_ss, n, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
+        else ebe_generic_kernel<KT, VT, true, false><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.ve_rs, a.ve_ss, n, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
     } else {
-        if (atomic) ebe_generic_kernel<KT, VT, false, true><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, n, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
-        else ebe_generic_kernel<KT, VT, false, false><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, n, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
+        if (atomic) ebe_generic_kernel<KT, VT, false, true><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.ve_rs, a.ve_ss, n, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
+        else ebe_generic_kernel<KT, VT, false, false><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.ve_rs, a.ve_ss, n, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
     }
 }
 
@@ -601,11 +612,12 @@ void launch_bf16_diag(const double* src_rowmajor, const double* k_ref, const dou
     AMX_HIP(hipGetLastError());
 }
 
-void launch_extract_diag(const double* K, const Layout& L, uint32_t n_slots, double* ve, hipStream_t s)
+void launch_extract_diag(const double* K, const Layout& L, uint32_t n_slots, uint32_t rs, uint32_t ss, double* ve,
+                         hipStream_t s)
 {
     const uint64_t rows = (uint64_t)n_slots * L.n;
     if (rows == 0) return;
-    extract_diag_kernel<<<blocks_for(rows, 256), 256, 0, s>>>(K, TiledIndex{L.n, L.G, L.P}, rows, ve);
+    extract_diag_kernel<<<blocks_for(rows, 256), 256, 0, s>>>(K, TiledIndex{L.n, L.G, L.P}, rows, rs, ss, ve);
     AMX_HIP(hipGetLastError());
 }
 

@@ -36,8 +36,10 @@ void launch_relayout_scaled(const double* k_ref, const double* scale, const uint
 // row-major matrices (src_rowmajor) or k_ref + scale.
 void launch_bf16_diag(const double* src_rowmajor, const double* k_ref, const double* scale, const uint32_t* slot_elem,
                       uint32_t n_slots, uint32_t n, float* out, hipStream_t s);
-// diagonal entries K_e(r,r) of every slot row from the tiled FP64 storage into ve.
-void launch_extract_diag(const double* K, const Layout& L, uint32_t n_slots, double* ve, hipStream_t s);
+// diagonal entries K_e(r,r) of every slot row from the tiled FP64 storage into ve
+// (row (slot, r) at ve[r*rs + slot*ss]).
+void launch_extract_diag(const double* K, const Layout& L, uint32_t n_slots, uint32_t rs, uint32_t ss, double* ve,
+                         hipStream_t s);
 
 // ---- matvec kernels ----------------------------------------------------
 struct EbeArgs {
@@ -46,6 +48,7 @@ struct EbeArgs {
     const void* u;
     void* out;              // ve (deterministic) or v (atomic)
     uint64_t ve_base;       // first ve row of this group
+    uint32_t ve_rs, ve_ss;  // element row (slot, r) at ve_base + r*ve_rs + slot*ve_ss
     uint32_t tile_begin, n_tiles;
     const uint32_t* skip = nullptr;  // device flag: nonzero -> kernel exits (converged solver)
     const float* kdiag = nullptr;    // BF16 storage: FP32 diagonal [slot][r]
@@ -70,6 +73,7 @@ struct EbeJob {
     const void* u;
     void* out;              // the job's ve array
     uint64_t ve_base;
+    uint32_t ve_rs, ve_ss;  // as EbeArgs
     uint32_t tile_begin, n_tiles;
     const uint32_t* skip;   // the job's converged flag
     const float* kdiag;
@@ -100,6 +104,7 @@ struct FreeArgs {
     const double* u;
     double* out;             // ve (deterministic) or v (atomic)
     uint64_t ve_base;
+    uint32_t ve_rs, ve_ss;   // as EbeArgs
     uint32_t slot_begin, n_slots, stride;
     const double* qp;        // [points][3*nodes+1]: dN/dxi per node, weight
     uint32_t nq;             // quadrature points (hex20: table padded to free_padded_points rows)

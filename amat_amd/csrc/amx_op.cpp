@@ -77,6 +77,8 @@ FreeArgs free_args(const Op& op, const Group& g, const void* u, void* out, uint3
     a.u = static_cast<const double*>(u);
     a.out = static_cast<double*>(out);
     a.ve_base = g.ve_base;
+    a.ve_rs = g.ve_rs;
+    a.ve_ss = g.ve_ss;
     a.slot_begin = slot_begin;
     a.n_slots = n_slots;
     a.stride = g.n_slots;
@@ -145,6 +147,8 @@ void finalize_op(Op& op)
         g.n_tiles = g.n_slots / g.G;
         g.ve_base = ve_rows;
         ve_rows += (uint64_t)g.n_slots * n;
+        g.ve_rs = g.G > 1 ? g.n_slots : 1u;
+        g.ve_ss = g.G > 1 ? 1u : n;
 
         // map [slot][r]
         std::vector<uint32_t> hmap((size_t)g.n_slots * n, 0);
@@ -219,8 +223,9 @@ void finalize_op(Op& op)
     for (const auto* list : {&op.indep_list, &op.dep_list}) {
         for (uint32_t e : *list) {
             const Group& g = op.groups[op.elem_group[e]];
-            const uint64_t base = g.ve_base + (uint64_t)g.elem_slot[op.elem_index[e]] * g.n;
-            for (uint32_t r = 0; r < g.n; r++) hpos[fill[op.local_map[op.eoff[e] + r]]++] = (uint32_t)(base + r);
+            const uint64_t base = g.ve_base + (uint64_t)g.elem_slot[op.elem_index[e]] * g.ve_ss;
+            for (uint32_t r = 0; r < g.n; r++)
+                hpos[fill[op.local_map[op.eoff[e] + r]]++] = (uint32_t)(base + (uint64_t)r * g.ve_rs);
         }
     }
     upload(op.off, hoff, s, op.device_bytes);
@@ -259,7 +264,8 @@ void finalize_op(Op& op)
         if (free_m)
             launch_free(op.elem_kind, false, false, true, free_args(op, g, nullptr, op.ve.p, 0, g.n_slots, nullptr), s);
         else
-            launch_extract_diag(g.K[0].as<double>(), g.L[0], g.n_slots, op.ve.as<double>() + g.ve_base, s);
+            launch_extract_diag(g.K[0].as<double>(), g.L[0], g.n_slots, g.ve_rs, g.ve_ss, op.ve.as<double>() + g.ve_base,
+                                s);
     }
 
     // ---- trace in element order (finalize_end :748-785)
@@ -271,12 +277,13 @@ void finalize_op(Op& op)
         AMX_HIP(hipStreamSynchronize(s));
         for (uint32_t e = 0; e < op.n_elem; e++) {
             const Group& g = op.groups[op.elem_group[e]];
-            const uint64_t base = g.ve_base + (uint64_t)g.elem_slot[op.elem_index[e]] * g.n;
+            const uint64_t base = g.ve_base + (uint64_t)g.elem_slot[op.elem_index[e]] * g.ve_ss;
             for (uint32_t r = 0; r < g.n; r++) {
+                const double dv = hve[base + (uint64_t)r * g.ve_rs];
                 // the diagonal kernel marks a non-positive Jacobian with NaN
-                AMX_REQUIRE(std::isfinite(hve[base + r]), AMX_INVALID_ARGUMENT,
+                AMX_REQUIRE(std::isfinite(dv), AMX_INVALID_ARGUMENT,
                             "element " + std::to_string(e) + ": non-positive Jacobian at a quadrature point");
-                tr += hve[base + r];
+                tr += dv;
             }
         }
     } else {
@@ -387,6 +394,8 @@ void ebe_launch(Op& op, KType kt, bool vt_double, bool masked, int phase, const 
         a.u = u;
         a.out = atomic ? v : ve;
         a.ve_base = g.ve_base;
+        a.ve_rs = g.ve_rs;
+        a.ve_ss = g.ve_ss;
         a.skip = skip;
         a.kdiag = kt == KType::BF16 ? g.kdiag16.as<float>() : nullptr;
         if (phase == 0) { a.tile_begin = 0; a.n_tiles = g.tiles_phase0; }

@@ -25,6 +25,10 @@ struct Group {
     std::vector<uint32_t> elem_slot;  // index into elems -> slot
     uint32_t n_slots = 0, n_tiles = 0, tiles_phase0 = 0;
     uint64_t ve_base = 0;             // first ve row
+    // element row (slot, r) at ve_base + r*ve_rs + slot*ve_ss: row-major per element for
+    // one-element tiles, row r of consecutive slots together for multi-element tiles (a
+    // wave of consecutive DoFs then reads neighbouring rows in the gather)
+    uint32_t ve_rs = 1, ve_ss = 1;
     DevBuf K[3];                      // tiled storage per KType (F64 always)
     DevBuf map;                       // [slot][r] local ids | identity-BC flag
     DevBuf kdiag16;                   // BF16 storage: FP32 corrected diagonal [slot][r]

@@ -165,6 +165,8 @@ void rebuild_tables(amx_server& S)
             a.u = j.p.p;
             a.out = j.ve.p;
             a.ve_base = grp.ve_base;
+            a.ve_rs = grp.ve_rs;
+            a.ve_ss = grp.ve_ss;
             a.tile_begin = 0;
             a.n_tiles = grp.n_tiles;
             a.skip = skip;
