@@ -157,10 +157,12 @@ template <typename VT> __device__ __forceinline__ void atomic_add(VT* p, VT x) {
 // Element row (slot, r) of the deterministic scatter is ve[ve_base + r*rs + slot*ss]:
 // rs = 1, ss = N (rows of an element together) or rs = n_slots, ss = 1 (row r of
 // consecutive elements together: what a wave of consecutive DoFs reads in the gather).
-template <int N, typename KT, typename VT, bool MASKED, bool ATOMIC>
-__device__ __forceinline__ void ebe_tile_body(const KT* __restrict__ K, const uint32_t* __restrict__ map,
-                                              const VT* __restrict__ u, VT* __restrict__ out, uint64_t ve_base,
-                                              uint32_t rs, uint32_t ss, uint32_t t, const float* __restrict__ kdiag)
+//
+// ebe_tile_acc: lane (g, r) of tile t forms row r of K_e u_e (m: its map entry).
+template <int N, typename KT, typename VT, bool MASKED>
+__device__ __forceinline__ VT ebe_tile_acc(const KT* __restrict__ K, const uint32_t* __restrict__ map,
+                                           const VT* __restrict__ u, uint32_t t, const float* __restrict__ kdiag,
+                                           uint32_t& m)
 {
     using AT = VT;
     constexpr int G = 64 / N;
@@ -172,7 +174,7 @@ __device__ __forceinline__ void ebe_tile_body(const KT* __restrict__ K, const ui
     const uint32_t lc = lane < (uint32_t)GN ? lane : (uint32_t)GN - 1u;
     const uint32_t g = lc / N;
     const uint64_t row = (uint64_t)t * GN + lc;  // = slot * N + r
-    const uint32_t m = map[row];
+    m = map[row];
     VT uv = u[m & kIdxMask];
     if (MASKED && (m & kFlagBit)) uv = VT(0);
     const AT ue = uv;
@@ -188,16 +190,34 @@ __device__ __forceinline__ void ebe_tile_body(const KT* __restrict__ K, const ui
     // BF16 storage keeps the diagonal in FP32, corrected for the row's
     // off-diagonal rounding (row sums preserved); the BF16 diagonal slot is 0
     if constexpr (std::is_same<KT, uint16_t>::value) acc = madd(acc, (AT)kdiag[row], ue);
+    return acc;
+}
+
+// one tile, stored by its own lanes
+template <int N, typename KT, typename VT, bool MASKED, bool ATOMIC>
+__device__ __forceinline__ void ebe_tile_body(const KT* __restrict__ K, const uint32_t* __restrict__ map,
+                                              const VT* __restrict__ u, VT* __restrict__ out, uint64_t ve_base,
+                                              uint32_t rs, uint32_t ss, uint32_t t, const float* __restrict__ kdiag)
+{
+    constexpr int G = 64 / N;
+    constexpr int GN = G * N;
+    uint32_t m;
+    const VT acc = ebe_tile_acc<N, KT, VT, MASKED>(K, map, u, t, kdiag, m);
+    const uint32_t lane = threadIdx.x & 63u;
     if (lane < (uint32_t)GN) {
         if constexpr (ATOMIC) {
             atomic_add(&out[m & kIdxMask], acc);
         } else {
-            const uint64_t slot = (uint64_t)t * G + g, r = lc - g * N;
+            const uint32_t g = lane / N;
+            const uint64_t slot = (uint64_t)t * G + g, r = lane - g * N;
             out[ve_base + r * rs + slot * ss] = acc;
         }
     }
 }
 
+// A block's 4 waves take 4 consecutive tiles.  Multi-element tiles with the
+// deterministic scatter (rows stored [r][slot]) stage the block's rows in LDS
+// and store each row index's 4G consecutive slots together.
 template <int N, typename KT, typename VT, bool MASKED, bool ATOMIC>
 __global__ __launch_bounds__(256) void ebe_tile_kernel(const KT* __restrict__ K, const uint32_t* __restrict__ map,
                                                       const VT* __restrict__ u, VT* __restrict__ out,
@@ -205,10 +225,33 @@ __global__ __launch_bounds__(256) void ebe_tile_kernel(const KT* __restrict__ K,
                                                       uint32_t n_tiles, const uint32_t* __restrict__ skip,
                                                       const float* __restrict__ kdiag)
 {
+    constexpr int G = 64 / N;
     if (skip && *skip) return;  // converged solver: nothing to do
-    const uint32_t tloc = xcd_block(blockIdx.x, gridDim.x) * 4u + (threadIdx.x >> 6);
-    if (tloc >= n_tiles) return;  // uniform per wave
-    ebe_tile_body<N, KT, VT, MASKED, ATOMIC>(K, map, u, out, ve_base, rs, ss, tile_begin + tloc, kdiag);
+    const uint32_t tb = xcd_block(blockIdx.x, gridDim.x) * 4u, w = threadIdx.x >> 6;
+    if constexpr (G == 1 || ATOMIC) {
+        if (tb + w >= n_tiles) return;  // uniform per wave
+        ebe_tile_body<N, KT, VT, MASKED, ATOMIC>(K, map, u, out, ve_base, rs, ss, tile_begin + tb + w, kdiag);
+    } else {
+        constexpr int GN = G * N;
+        __shared__ VT st[N * 4 * G];  // [r][w*G + g]
+        if (tb >= n_tiles) return;    // uniform per block
+        const uint32_t nt = min(4u, n_tiles - tb), lane = threadIdx.x & 63u;
+        if (w < nt) {
+            uint32_t m;
+            const VT acc = ebe_tile_acc<N, KT, VT, MASKED>(K, map, u, tile_begin + tb + w, kdiag, m);
+            if (lane < (uint32_t)GN) {
+                const uint32_t g = lane / N, r = lane - g * N;
+                st[r * (4 * G) + w * G + g] = acc;
+            }
+        }
+        __syncthreads();
+        const uint32_t per_r = nt * G;
+        const uint64_t slot0 = (uint64_t)(tile_begin + tb) * G;
+        for (uint32_t i = threadIdx.x; i < (uint32_t)N * per_r; i += 256u) {
+            const uint32_t r = i / per_r, k = i - r * per_r;
+            out[ve_base + (uint64_t)r * rs + (slot0 + k) * ss] = st[r * (4 * G) + k];
+        }
+    }
 }
 
 // index of the entry that owns work item x: the largest j < n with prefix[j] <= x
