@@ -283,3 +283,14 @@ def test_c2_full_size_bitexact(torch_gpu):
     w = mesh.uniform(77, 0, p.n_dofs, -1, 1)
     kw = run(torch_gpu, op.matvec_dev, w)
     assert abs(np.dot(w, v) - np.dot(u, kw)) <= 1e-12 * np.linalg.norm(w) * np.linalg.norm(v)
+
+
+def test_measured_read_bandwidth(torch_gpu):
+    """The measured HBM read stream used beside the nominal 8 TB/s peak: a
+    plausible figure for one MI355X, and argument checks."""
+    from amat_amd.amat import measure_read_bandwidth
+    from amat_amd import AmatError
+    bw = measure_read_bandwidth(torch_gpu.cuda.current_device(), 1 << 30, 3)
+    assert 1e12 < bw < 1e13
+    with pytest.raises(AmatError):
+        measure_read_bandwidth(0, 1024, 1)
