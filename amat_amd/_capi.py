@@ -31,6 +31,19 @@ class MapsInfo(C.Structure):
         ("owned_begin", C.c_uint64), ("owned_end", C.c_uint64), ("n_global", C.c_uint64)]
 
 
+class GmshInfo(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in (
+        "rank", "size", "dofs_per_node", "n_owned_elements", "n_owned_nodes", "n_pre_ghost_nodes",
+        "n_post_ghost_nodes", "n_local_nodes", "n_entries", "max_nodes_per_element", "n_boundary_nodes",
+        "n_partitions")] + [(n, C.c_uint64) for n in (
+            "n_nodes", "n_file_nodes", "n_volume_elements", "n_surface_elements", "n_line_elements",
+            "n_point_elements", "node_offset", "n_dofs_total", "owned_dof_begin", "owned_dof_end")]
+
+
+# displacement(xyz, values) callback of amx_gmsh_get_prescribed_values
+DISPLACEMENT_FN = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+
 class ServerInfo(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("active_jobs", "held_jobs", "free_slots", "pad")] + [
         (n, C.c_uint64) for n in ("workspace_bytes", "steps", "iterations", "completed")]
@@ -132,6 +145,16 @@ SIGNATURES = {
     "amx_fem_fe_tet10": (I32, [P, P, U32, P]),
     "amx_fem_ke_tet10_batch": (I32, [U32, P, U32, P]),
     "amx_fem_fe_tet10_batch": (I32, [U32, P, P, U32, P]),
+    "amx_gmsh_read": (I32, [C.c_char_p, U32, U32, U32, C.POINTER(P)]),
+    "amx_gmsh_destroy": (I32, [P]),
+    "amx_gmsh_get_info": (I32, [P, C.POINTER(GmshInfo)]),
+    "amx_gmsh_get_element_sizes": (I32, [P, P, P, P]),
+    "amx_gmsh_get_maps": (I32, [P, P, P, P, P]),
+    "amx_gmsh_get_local2global": (I32, [P, P, P]),
+    "amx_gmsh_get_rank_node_counts": (I32, [P, P]),
+    "amx_gmsh_get_coordinates": (I32, [P, P]),
+    "amx_gmsh_get_constraints": (I32, [P, P, P]),
+    "amx_gmsh_get_prescribed_values": (I32, [P, DISPLACEMENT_FN, P]),
 }
 
 _lib: C.CDLL | None = None

@@ -34,6 +34,7 @@ SOURCES = [
     "amx_op.cpp",
     "amx_solver.cpp",
     "amx_serve.cpp",
+    "amx_gmsh.cpp",
     "amx_kernels.hip",
     "amx_solver_kernels.hip",
     "amx_mrhs.hip",

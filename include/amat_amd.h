@@ -450,6 +450,72 @@ int amx_fem_ke_tet10_batch(uint32_t n_elem, const double* coords, uint32_t ngt, 
 int amx_fem_fe_tet10_batch(uint32_t n_elem, const double* coords, const double* be,
                            uint32_t ngt, double* fe);
 
+/* ------------------------------------------------------------ Gmsh ingest
+ * GmshMesh (examples/include/GmshMesh.hpp, examples/src/GmshMesh.cpp:133-804):
+ * an ASCII Gmsh 2.x mesh, partitioned by Gmsh, as one rank's inputs of
+ * amx_maps_set_map / amx_maps_set_bdr_map.  Every rank reads the whole file.
+ * Volume elements (Gmsh types 4 tet4, 5 hex8, 11 tet10, 12 hex27, 17 hex20)
+ * are the mesh; surface elements (2, 3, 9, 10, 16) give the constrained nodes;
+ * points (15) and lines (1, 8) are skipped.  Element nodes come in the aMat
+ * element orders (tet10: Gmsh nodes 8 and 9 swapped, GmshMesh.cpp:364-377;
+ * hex20: examples/include/shapeFunc.hpp:200-269).  A node is owned by the
+ * partition of the first volume element, in file order, that names it
+ * (GmshMesh.cpp:379-466); a rank's owned nodes get consecutive global ids in
+ * ascending Gmsh id order (:480-495).  Local node ids are
+ * [pre ghosts | owned | post ghosts]. */
+typedef struct amx_gmsh amx_gmsh;
+
+typedef struct amx_gmsh_info {
+    uint32_t rank, size, dofs_per_node;
+    uint32_t n_owned_elements;      /* GmshMesh::get_nOwnedElems */
+    uint32_t n_owned_nodes;         /* get_nOwnedDofs / dofs_per_node */
+    uint32_t n_pre_ghost_nodes, n_post_ghost_nodes;
+    uint32_t n_local_nodes;         /* get_nLocalNodes */
+    uint32_t n_entries;             /* sum of the owned elements' node counts */
+    uint32_t max_nodes_per_element;
+    uint32_t n_boundary_nodes;      /* get_n_constraints / dofs_per_node */
+    uint32_t n_partitions;          /* highest partition id of the file (1 if none) */
+    uint64_t n_nodes;               /* nodes of the volume elements, all ranks */
+    uint64_t n_file_nodes;          /* nodes listed in $Nodes */
+    uint64_t n_volume_elements, n_surface_elements, n_line_elements, n_point_elements;
+    uint64_t node_offset;           /* first global node id of this rank */
+    uint64_t n_dofs_total;          /* get_nDofsTotal */
+    uint64_t owned_dof_begin, owned_dof_end; /* get_startGlobalDof / get_endGlobalDof (inclusive) */
+} amx_gmsh_info;
+
+/* GmshMesh(comm, nDofPerNode) + setGmshMesh (GmshMesh.cpp:133-331).  size > 1
+ * needs partition tags on every element; partition ids must be <= size. */
+int amx_gmsh_read(const char* path, uint32_t rank, uint32_t size, uint32_t dofs_per_node,
+                  amx_gmsh** out);
+int amx_gmsh_destroy(amx_gmsh* mesh);
+int amx_gmsh_get_info(const amx_gmsh* mesh, amx_gmsh_info* info);
+/* per owned element (n_owned_elements): get_nNodesPerElem / get_nDofsPerElem
+ * (either may be NULL) and the Gmsh element type */
+int amx_gmsh_get_element_sizes(const amx_gmsh* mesh, uint32_t* nodes_per_element,
+                               uint32_t* dofs_per_element, uint32_t* gmsh_types);
+/* element maps flattened element after element: node level (n_entries,
+ * get_localMap / get_globalMap) and DoF level (n_entries * dofs_per_node,
+ * get_localDofMap / get_globalDofMap, DoF = node * dofs_per_node + component);
+ * any output may be NULL */
+int amx_gmsh_get_maps(const amx_gmsh* mesh, uint32_t* local_map, uint64_t* global_map,
+                      uint32_t* local_dof_map, uint64_t* global_dof_map);
+/* local node -> global node (n_local_nodes, get_local2GlobalMap) and local DoF
+ * -> global DoF (n_local_nodes * dofs_per_node, get_local2GlobalDofMap) */
+int amx_gmsh_get_local2global(const amx_gmsh* mesh, uint64_t* nodes, uint64_t* dofs);
+/* owned node count of every rank (size entries); times dofs_per_node these are
+ * the rank_dof_counts of amx_maps_set_map */
+int amx_gmsh_get_rank_node_counts(const amx_gmsh* mesh, uint32_t* counts);
+/* x, y, z of every local node (n_local_nodes * 3; get_x / get_y / get_z) */
+int amx_gmsh_get_coordinates(const amx_gmsh* mesh, double* xyz);
+/* boundary nodes (n_boundary_nodes global node ids, ascending) and the
+ * constrained DoFs (n_boundary_nodes * dofs_per_node; get_constrained_dofs) */
+int amx_gmsh_get_constraints(const amx_gmsh* mesh, uint64_t* boundary_nodes, uint64_t* constrained_dofs);
+/* get_prescribed_values (GmshMesh.cpp:752-783): displacement(xyz, values)
+ * writes dofs_per_node values for a boundary node at xyz. */
+int amx_gmsh_get_prescribed_values(const amx_gmsh* mesh,
+                                   void (*displacement)(const double* xyz, double* values),
+                                   double* values);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,11 +4,14 @@
 //
 // Test program: built by amat_amd/build.py, run by tests/test_gpu_dropin.py.
 // Prints one line "dropin <case> maxdiff_matvec=... maxdiff_mf=..." per case
-// and exits non-zero on any mismatch.
+// and exits non-zero on any mismatch.  With a Gmsh file as argument it also
+// runs the set-up of examples/src/example05a.cpp:135-246 (tet10 Laplace read
+// by GmshMesh) on one rank.
 #include <cmath>
 #include <cstdio>
 #include <vector>
 
+#include "amat_amd/GmshMesh.hpp"
 #include "amat_amd/aMatFree.hpp"
 #include "amx_oracle.h"
 
@@ -177,10 +180,86 @@ static int run_free_case(unsigned Nex, unsigned Ney, unsigned Nez)
     return (da == 0.0 && db <= 1e-13 * nr) ? 0 : 1;
 }
 
-int main()
+struct RowView {  // a row-major n x n block
+    const double* a;
+    long n;
+    long rows() const { return n; }
+    long cols() const { return n; }
+    double operator()(long r, long c) const { return a[r * n + c]; }
+};
+
+static void zero_displacement(const double*, double* uvw) { *uvw = 0.0; }
+
+static int run_gmsh_case(const char* path)
+{
+    using Maps = par::Maps<double, unsigned long, unsigned int>;
+    using AFM = par::aMatFree<double, unsigned long, unsigned int>;
+    GmshMesh mesh(0, 1, 1);
+    mesh.setGmshMesh(path);
+    const unsigned int nelem_owned = mesh.get_nOwnedElems();
+    unsigned int** localDofMap = mesh.get_localDofMap();
+    unsigned int* nDofsPerElem = mesh.get_nDofsPerElem();
+    const unsigned int nLocalDofs = mesh.get_nLocalDofs();
+    unsigned long* local2GlobalDofMap = mesh.get_local2GlobalDofMap();
+    const unsigned long startGlobalDof = mesh.get_startGlobalDof();
+    const unsigned long endGlobalDof = mesh.get_endGlobalDof();
+    const unsigned long nDofsTotal = mesh.get_nDofsTotal();
+    const unsigned int nConstraints = mesh.get_n_constraints();
+    unsigned long* constrainedDofs = mesh.get_constrained_dofs();
+    double* prescribedValues = mesh.get_prescribed_values(&zero_displacement);
+    unsigned int* nNodesPerElem = mesh.get_nNodesPerElem();
+    unsigned int** localMap = mesh.get_localMap();
+
+    Maps meshMaps;
+    meshMaps.set_map(nelem_owned, localDofMap, nDofsPerElem, nLocalDofs, local2GlobalDofMap, startGlobalDof,
+                     endGlobalDof, nDofsTotal);
+    meshMaps.set_bdr_map(constrainedDofs, prescribedValues, nConstraints);
+    AFM A(meshMaps, par::BC_METH::BC_IMATRIX);
+    A.set_matfree_type(par::MATFREE_TYPE::GPU_PURE);
+    std::vector<double> Kall((size_t)nelem_owned * 100);
+    std::vector<unsigned> flat, order(nelem_owned);
+    for (unsigned eid = 0; eid < nelem_owned; eid++) {
+        if (nNodesPerElem[eid] != 10) {
+            std::printf("dropin gmsh: element %u has %u nodes\n", eid, nNodesPerElem[eid]);
+            return 1;
+        }
+        double xe[30];
+        for (unsigned nid = 0; nid < 10; nid++) {
+            const unsigned localNid = localMap[eid][nid];
+            xe[nid * 3] = mesh.get_x(localNid);
+            xe[nid * 3 + 1] = mesh.get_y(localNid);
+            xe[nid * 3 + 2] = mesh.get_z(localNid);
+        }
+        par::detail::check(amx_fem_ke_tet10(xe, 2, &Kall[(size_t)eid * 100]), "ke_tet10");
+        A.set_element_matrix(eid, RowView{&Kall[(size_t)eid * 100], 10}, 0, 0, 1);
+        flat.insert(flat.end(), localDofMap[eid], localDofMap[eid] + 10);
+        order[eid] = eid;
+    }
+    A.finalize();
+    std::vector<double> u(nLocalDofs), v(nLocalDofs), vref(nLocalDofs), vm(nLocalDofs), vmref(nLocalDofs);
+    for (unsigned i = 0; i < nLocalDofs; i++) u[i] = std::sin(0.41 * i) + 0.2 * std::cos(0.9 * i);
+    A.matvec(v.data(), u.data(), true);
+    A.matmult_mf(vm.data(), u.data());
+    orc_matvec_ghosted_omp(nelem_owned, nDofsPerElem, flat.data(), Kall.data(), 0, order.data(), nelem_owned, nullptr,
+                           0, u.data(), vref.data(), nLocalDofs, 1);
+    std::vector<uint64_t> oc(constrainedDofs, constrainedDofs + nConstraints);
+    orc_matmult_mf(nelem_owned, nDofsPerElem, flat.data(), Kall.data(), nLocalDofs, 0, nLocalDofs, oc.data(),
+                   (uint32_t)oc.size(), 0, u.data(), vmref.data(), 1);
+    double d1 = 0, d2 = 0;
+    for (unsigned i = 0; i < nLocalDofs; i++) {
+        d1 = std::fmax(d1, std::fabs(v[i] - vref[i]));
+        d2 = std::fmax(d2, std::fabs(vm[i] - vmref[i]));
+    }
+    std::printf("dropin gmsh_tet10_%u maxdiff_matvec=%.3e maxdiff_mf=%.3e constraints=%u\n", nelem_owned, d1, d2,
+                nConstraints);
+    return (d1 == 0.0 && d2 == 0.0) ? 0 : 1;
+}
+
+int main(int argc, char** argv)
 {
     int bad = 0;
     try {
+        if (argc > 1) bad |= run_gmsh_case(argv[1]);
         bad |= run_case(4, 5, 6);
         bad |= run_case(16, 16, 16);
         bad |= run_free_case(7, 6, 5);

@@ -11,6 +11,10 @@ Run in the build container only (it needs the reference tree and MPICH):
 * maps_<case>_p<P>.npz -- inputs of par::Maps::set_map / set_bdr_map for every
   rank of a z-slab partition (built by amat_amd.mesh) and the outputs of the
   reference include/maps.hpp run under `mpiexec -n P` (oracle/_ref/ref_maps_driver).
+* gmsh_<case>.npz -- an ASCII Gmsh 2.2 file of a structured mesh with partition
+  tags (written by amat_amd.gmsh.write_msh) and, for every rank, the getters of
+  the reference examples/src/GmshMesh.cpp run on it under `mpiexec -n P`
+  (oracle/_ref/ref_gmsh_driver).
 
 The fixtures are data only: inputs and the reference's outputs.
 """
@@ -161,12 +165,55 @@ def maps_fixtures() -> None:
         maps_case("tet10", t10, P, meshmod.boundary_nodes(t10))
 
 
+def gmsh_case(name: str, m: meshmod.Mesh, P: int, dpn: int, parts) -> None:
+    from amat_amd import gmsh as gmshmod
+    with tempfile.TemporaryDirectory() as td:
+        tdp = pathlib.Path(td)
+        path = tdp / "mesh.msh"
+        gmshmod.write_msh(path, m, parts=parts)
+        cmd = [MPIEXEC, "-launcher", "fork", "-n", str(P), str(REF / "ref_gmsh_driver"), str(path), str(dpn),
+               str(tdp)]
+        subprocess.run(cmd, check=True, capture_output=True, timeout=300)
+        rec = {"msh": np.frombuffer(path.read_bytes(), dtype=np.uint8), "size": np.array([P]),
+               "dofs_per_node": np.array([dpn])}
+        names = ["scalars", "npe", "ndpe", "local_map", "global_map", "local_dof_map", "global_dof_map",
+                 "l2g_at_entries", "l2gdof_at_entries", "xyz_at_entries", "constrained_dofs", "prescribed"]
+        types = [np.uint64, np.uint32, np.uint32, np.uint32, np.uint64, np.uint32, np.uint64, np.uint64, np.uint64,
+                 np.float64, np.uint64, np.float64]
+        for r in range(P):
+            with open(tdp / f"gmsh_r{r}.bin", "rb") as f:
+                for nm, ty in zip(names, types):
+                    rec[f"r{r}_{nm}"] = _read_vec(f, ty)
+    np.savez_compressed(OUT / f"gmsh_{name}.npz", **rec)
+    print(f"wrote gmsh_{name}.npz")
+
+
+def gmsh_fixtures() -> None:
+    rng = np.random.default_rng(20241202)
+
+    def scatter(E, P):
+        parts = rng.integers(0, P, E)
+        parts[:P] = np.arange(P)          # no empty partition
+        return rng.permutation(parts)
+
+    h8 = meshmod.hex8(3, 3, 4)
+    layer = np.arange(h8.n_elem) // 9
+    gmsh_case("hex8_p1", h8, 1, 3, None)
+    gmsh_case("hex8_p2_slab", h8, 2, 3, (layer >= 2).astype(int))
+    gmsh_case("hex8_p3_scatter", h8, 3, 3, scatter(h8.n_elem, 3))
+    t10 = meshmod.tet10(2, seed=5)
+    gmsh_case("tet10_p1", t10, 1, 1, None)
+    gmsh_case("tet10_p2_scatter", t10, 2, 1, scatter(t10.n_elem, 2))
+    gmsh_case("tet10_p4_blocks", t10, 4, 1, np.arange(t10.n_elem) * 4 // t10.n_elem)
+
+
 def main() -> None:
     subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), "ref"], check=True)
     if shutil.which(MPIEXEC) is None and not pathlib.Path(MPIEXEC).exists():
         raise SystemExit("MPICH mpiexec not found")
     fem_fixtures()
     maps_fixtures()
+    gmsh_fixtures()
 
 
 if __name__ == "__main__":
