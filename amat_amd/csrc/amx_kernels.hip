@@ -215,10 +215,12 @@ __device__ __forceinline__ void ebe_tile_body(const KT* __restrict__ K, const ui
     }
 }
 
-// A block's 4 waves take 4 consecutive tiles.  Multi-element tiles with the
-// deterministic scatter (rows stored [r][slot]) stage the block's rows in LDS
-// and store each row index's 4G consecutive slots together.
-template <int N, typename KT, typename VT, bool MASKED, bool ATOMIC>
+// A block's 4 waves take 4 consecutive tiles (TPW consecutive tiles per wave
+// for multi-element tiles with the deterministic scatter: their loads are in
+// flight together).  Multi-element tiles with the deterministic scatter (rows
+// stored [r][slot]) stage the block's rows in LDS and store each row index's
+// consecutive slots together.
+template <int N, typename KT, typename VT, bool MASKED, bool ATOMIC, int TPW>
 __global__ __launch_bounds__(256) void ebe_tile_kernel(const KT* __restrict__ K, const uint32_t* __restrict__ map,
                                                       const VT* __restrict__ u, VT* __restrict__ out,
                                                       uint64_t ve_base, uint32_t rs, uint32_t ss, uint32_t tile_begin,
@@ -227,31 +229,53 @@ __global__ __launch_bounds__(256) void ebe_tile_kernel(const KT* __restrict__ K,
 {
     constexpr int G = 64 / N;
     if (skip && *skip) return;  // converged solver: nothing to do
-    const uint32_t tb = xcd_block(blockIdx.x, gridDim.x) * 4u, w = threadIdx.x >> 6;
+    const uint32_t w = threadIdx.x >> 6;
     if constexpr (G == 1 || ATOMIC) {
+        const uint32_t tb = xcd_block(blockIdx.x, gridDim.x) * 4u;
         if (tb + w >= n_tiles) return;  // uniform per wave
         ebe_tile_body<N, KT, VT, MASKED, ATOMIC>(K, map, u, out, ve_base, rs, ss, tile_begin + tb + w, kdiag);
     } else {
         constexpr int GN = G * N;
-        __shared__ VT st[N * 4 * G];  // [r][w*G + g]
-        if (tb >= n_tiles) return;    // uniform per block
-        const uint32_t nt = min(4u, n_tiles - tb), lane = threadIdx.x & 63u;
-        if (w < nt) {
-            uint32_t m;
-            const VT acc = ebe_tile_acc<N, KT, VT, MASKED>(K, map, u, tile_begin + tb + w, kdiag, m);
-            if (lane < (uint32_t)GN) {
-                const uint32_t g = lane / N, r = lane - g * N;
-                st[r * (4 * G) + w * G + g] = acc;
-            }
+        constexpr uint32_t BT = 4u * TPW;  // tiles per block
+        __shared__ VT st[N * BT * G];      // [r][tile in block * G + g]
+        const uint32_t tb = xcd_block(blockIdx.x, gridDim.x) * BT;
+        if (tb >= n_tiles) return;         // uniform per block
+        const uint32_t nt = min(BT, n_tiles - tb), lane = threadIdx.x & 63u;
+        VT acc[TPW];
+        uint32_t m;
+        if (nt == BT) {
+#pragma unroll
+            for (int i = 0; i < TPW; i++)
+                acc[i] = ebe_tile_acc<N, KT, VT, MASKED>(K, map, u, tile_begin + tb + w * TPW + i, kdiag, m);
+        } else {
+#pragma unroll
+            for (int i = 0; i < TPW; i++)
+                if (w * TPW + i < nt)
+                    acc[i] = ebe_tile_acc<N, KT, VT, MASKED>(K, map, u, tile_begin + tb + w * TPW + i, kdiag, m);
+        }
+        if (lane < (uint32_t)GN) {
+            const uint32_t g = lane / N, r = lane - g * N;
+#pragma unroll
+            for (int i = 0; i < TPW; i++)
+                if (w * TPW + i < nt) st[r * (BT * G) + (w * TPW + i) * G + g] = acc[i];
         }
         __syncthreads();
         const uint32_t per_r = nt * G;
         const uint64_t slot0 = (uint64_t)(tile_begin + tb) * G;
         for (uint32_t i = threadIdx.x; i < (uint32_t)N * per_r; i += 256u) {
             const uint32_t r = i / per_r, k = i - r * per_r;
-            out[ve_base + (uint64_t)r * rs + (slot0 + k) * ss] = st[r * (4 * G) + k];
+            out[ve_base + (uint64_t)r * rs + (slot0 + k) * ss] = st[r * (BT * G) + k];
         }
     }
+}
+
+// tiles per wave of the element kernel: two for multi-element tiles of FP32 /
+// BF16 storage (measured: tet10 96^3 BF16 matvec 506 -> 476 us, hex8 128^3 122 ->
+// 109 us; FP32 641 -> 629 and 142 -> 136 us), one for FP64 (two were 2-4 % slower)
+// and for one-element tiles
+template <int N, typename KT> constexpr int tiles_per_wave()
+{
+    return (64 / N == 1 || KSize<KT>::v == 8) ? 1 : 2;
 }
 
 // index of the entry that owns work item x: the largest j < n with prefix[j] <= x
@@ -540,17 +564,23 @@ inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs 
 template <int N, typename KT, typename VT>
 void launch_tile(bool masked, bool atomic, const EbeArgs& a, hipStream_t s)
 {
-    const dim3 grid(blocks_for(a.n_tiles, 4)), block(256);
+    constexpr int TPW = tiles_per_wave<N, KT>();
+    const uint32_t per_block = atomic ? 4u : 4u * TPW;
+    const dim3 grid(blocks_for(a.n_tiles, per_block)), block(256);
     const KT* K = static_cast<const KT*>(a.K);
     const VT* u = static_cast<const VT*>(a.u);
     VT* out = static_cast<VT*>(a.out);
+#define AMX_TILE(M, A, T)                                                                                          \
+    ebe_tile_kernel<N, KT, VT, M, A, T><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.ve_rs, a.ve_ss,      \
+                                                                a.tile_begin, a.n_tiles, a.skip, a.kdiag)
     if (masked) {
-        if (atomic) ebe_tile_kernel<N, KT, VT, true, true><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.ve_rs, a.ve_ss, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
-        else ebe_tile_kernel<N, KT, VT, true, false><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.ve_rs, a.ve_ss, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
+        if (atomic) AMX_TILE(true, true, 1);
+        else AMX_TILE(true, false, TPW);
     } else {
-        if (atomic) ebe_tile_kernel<N, KT, VT, false, true><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.ve_rs, a.ve_ss, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
-        else ebe_tile_kernel<N, KT, VT, false, false><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.ve_rs, a.ve_ss, a.tile_begin, a.n_tiles, a.skip, a.kdiag);
+        if (atomic) AMX_TILE(false, true, 1);
+        else AMX_TILE(false, false, TPW);
     }
+#undef AMX_TILE
 }
 
 template <typename KT, typename VT>
