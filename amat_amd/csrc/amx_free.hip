@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <type_traits>
 
 #include "amx_internal.hpp"
 #include "amx_kernels.hpp"
@@ -44,15 +45,26 @@ __device__ __forceinline__ void lds_sync_wave()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// two-value LDS accesses of the compute type RT (FP64 or FP32)
+template <typename RT> struct Vec2;
+template <> struct Vec2<double> {
+    using type = double2;
+    __device__ static type make(double a, double b) { return make_double2(a, b); }
+};
+template <> struct Vec2<float> {
+    using type = float2;
+    __device__ static type make(float a, float b) { return make_float2(a, b); }
+};
+
 // det J and inv[d][k] = cofactor(J)[k][d] / det (cofactors as in the host
 // element routines, amx_fem.cpp jacobian; one division)
-__device__ __forceinline__ double jac_inv_r(const double J[3][3], double inv[3][3])
+template <typename RT> __device__ __forceinline__ RT jac_inv_r(const RT J[3][3], RT inv[3][3])
 {
-    const double c00 = J[1][1] * J[2][2] - J[2][1] * J[1][2];
-    const double c10 = J[2][1] * J[0][2] - J[0][1] * J[2][2];
-    const double c20 = J[0][1] * J[1][2] - J[1][1] * J[0][2];
-    const double det = J[0][0] * c00 + J[1][0] * c10 + J[2][0] * c20;
-    const double r = 1.0 / det;
+    const RT c00 = J[1][1] * J[2][2] - J[2][1] * J[1][2];
+    const RT c10 = J[2][1] * J[0][2] - J[0][1] * J[2][2];
+    const RT c20 = J[0][1] * J[1][2] - J[1][1] * J[0][2];
+    const RT det = J[0][0] * c00 + J[1][0] * c10 + J[2][0] * c20;
+    const RT r = RT(1) / det;
     inv[0][0] = c00 * r;
     inv[0][1] = c10 * r;
     inv[0][2] = c20 * r;
@@ -68,10 +80,13 @@ __device__ __forceinline__ double jac_inv_r(const double J[3][3], double inv[3][
 // ---- Laplace elements: one lane per element slot ------------------------
 // xe: [NN*3][stride] (coordinate-major, slot fastest), qp: [nq][NN*3+1]
 // (dN_ad at 3a+d, weight last; uniform across lanes -> scalar loads).
-template <int NN, bool MASKED, bool ATOMIC, bool DIAG>
+// RT: compute and vector type (FP64; FP32 for the inner solves of the
+// mixed-precision solvers).  Coordinates, factors and the table stay FP64 in
+// memory and are rounded on load.
+template <int NN, typename RT, bool MASKED, bool ATOMIC, bool DIAG>
 __global__ __launch_bounds__(256) void free_laplace_kernel(const double* __restrict__ xe, const double* __restrict__ scale,
-                                                          const uint32_t* __restrict__ map, const double* __restrict__ u,
-                                                          double* __restrict__ out, uint64_t ve_base, uint32_t rs,
+                                                          const uint32_t* __restrict__ map, const RT* __restrict__ u,
+                                                          RT* __restrict__ out, uint64_t ve_base, uint32_t rs,
                                                           uint32_t ss, uint32_t slot_begin, uint32_t n_slots,
                                                           uint32_t stride, const double* __restrict__ qp, uint32_t nq,
                                                           const uint32_t* __restrict__ skip)
@@ -80,58 +95,58 @@ __global__ __launch_bounds__(256) void free_laplace_kernel(const double* __restr
     const uint32_t sl = blockIdx.x * blockDim.x + threadIdx.x;
     if (sl >= n_slots) return;
     const uint64_t slot = (uint64_t)slot_begin + sl;
-    double x[NN * 3], ue[NN], f[NN];
+    RT x[NN * 3], ue[NN], f[NN];
     uint32_t m[NN];
 #pragma unroll
-    for (int i = 0; i < NN * 3; i++) x[i] = xe[(uint64_t)i * stride + slot];
+    for (int i = 0; i < NN * 3; i++) x[i] = (RT)xe[(uint64_t)i * stride + slot];
 #pragma unroll
     for (int a = 0; a < NN; a++) {
         m[a] = map[slot * NN + a];
-        ue[a] = 0.0;
+        ue[a] = RT(0);
         if constexpr (!DIAG) {
-            const double v = u[m[a] & kMask];
-            ue[a] = (MASKED && (m[a] & kFlag)) ? 0.0 : v;
+            const RT v = u[m[a] & kMask];
+            ue[a] = (MASKED && (m[a] & kFlag)) ? RT(0) : v;
         }
-        f[a] = 0.0;
+        f[a] = RT(0);
     }
-    const double sc = scale[slot];
+    const RT sc = (RT)scale[slot];
     for (uint32_t q = 0; q < nq; q++) {
         const double* t = qp + (uint64_t)q * (NN * 3 + 1);
-        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, h[3] = {0, 0, 0};
+        RT J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, h[3] = {0, 0, 0};
 #pragma unroll
         for (int a = 0; a < NN; a++)
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                const double dn = t[3 * a + d];
+                const RT dn = (RT)t[3 * a + d];
 #pragma unroll
                 for (int k = 0; k < 3; k++) J[k][d] += x[3 * a + k] * dn;
                 if constexpr (!DIAG) h[d] += ue[a] * dn;
             }
-        double inv[3][3];
-        const double det = jac_inv_r(J, inv);
-        double w = det * t[NN * 3] * sc;
+        RT inv[3][3];
+        const RT det = jac_inv_r(J, inv);
+        RT w = det * (RT)t[NN * 3] * sc;
         if constexpr (DIAG) {
             // a non-positive Jacobian at any point marks the element (NaN diagonal);
             // finalize rejects it, as the element routines do
-            if (!(det > 0.0)) w = __builtin_nan("");
+            if (!(det > RT(0))) w = (RT)__builtin_nan("");
 #pragma unroll
             for (int a = 0; a < NN; a++) {
-                double s = 0.0;
+                RT s = RT(0);
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
-                    const double g = t[3 * a] * inv[0][k] + t[3 * a + 1] * inv[1][k] + t[3 * a + 2] * inv[2][k];
+                    const RT g = (RT)t[3 * a] * inv[0][k] + (RT)t[3 * a + 1] * inv[1][k] + (RT)t[3 * a + 2] * inv[2][k];
                     s += g * g;
                 }
                 f[a] += s * w;
             }
         } else {
-            double gu[3], p[3];
+            RT gu[3], p[3];
 #pragma unroll
             for (int k = 0; k < 3; k++) gu[k] = (h[0] * inv[0][k] + h[1] * inv[1][k] + h[2] * inv[2][k]) * w;
 #pragma unroll
             for (int d = 0; d < 3; d++) p[d] = inv[d][0] * gu[0] + inv[d][1] * gu[1] + inv[d][2] * gu[2];
 #pragma unroll
-            for (int a = 0; a < NN; a++) f[a] += t[3 * a] * p[0] + t[3 * a + 1] * p[1] + t[3 * a + 2] * p[2];
+            for (int a = 0; a < NN; a++) f[a] += (RT)t[3 * a] * p[0] + (RT)t[3 * a + 1] * p[1] + (RT)t[3 * a + 2] * p[2];
         }
     }
 #pragma unroll
@@ -162,29 +177,31 @@ template <int QP> constexpr int h20_wave_doubles()
     return (64 / QP) * kElemRow > 64 * kH20Slot ? (64 / QP) * kElemRow : 64 * kH20Slot;
 }
 
-template <int QP, bool MASKED, bool ATOMIC, bool DIAG>
+template <int QP, typename RT, bool MASKED, bool ATOMIC, bool DIAG>
 __global__ __launch_bounds__(256) void free_hex20_kernel(const double* __restrict__ xe, const double* __restrict__ scale,
-                                                        const uint32_t* __restrict__ map, const double* __restrict__ u,
-                                                        double* __restrict__ out, uint64_t ve_base, uint32_t rs,
+                                                        const uint32_t* __restrict__ map, const RT* __restrict__ u,
+                                                        RT* __restrict__ out, uint64_t ve_base, uint32_t rs,
                                                         uint32_t ss, uint32_t slot_begin, uint32_t n_slots,
                                                         const double* __restrict__ qp, uint32_t nq,
                                                         double c1, double c2, double c3,
                                                         const uint32_t* __restrict__ skip)
 {
+    using RT2 = typename Vec2<RT>::type;
     constexpr int EW = 64 / QP, WD = h20_wave_doubles<QP>();
-    __shared__ __align__(16) double tab[QP * kTabRow];
-    __shared__ double wq[QP];
-    __shared__ __align__(16) double work[4][WD];
+    __shared__ __align__(16) RT tab[QP * kTabRow];
+    __shared__ RT wq[QP];
+    __shared__ __align__(16) RT work[4][WD];
+    const RT k1 = (RT)c1, k2 = (RT)c2, k3 = (RT)c3;
     if (skip && *skip) return;
     for (uint32_t i = threadIdx.x; i < QP * 80u; i += blockDim.x) {
         const uint32_t q = i / 80u, r = i - q * 80u, a = r >> 2, d = r & 3u;
-        tab[q * kTabRow + r] = d < 3 ? qp[q * 61u + 3u * a + d] : 0.0;
+        tab[q * kTabRow + r] = d < 3 ? (RT)qp[q * 61u + 3u * a + d] : RT(0);
     }
-    for (uint32_t q = threadIdx.x; q < QP; q += blockDim.x) wq[q] = qp[q * 61u + 60u];
+    for (uint32_t q = threadIdx.x; q < QP; q += blockDim.x) wq[q] = (RT)qp[q * 61u + 60u];
     __syncthreads();
 
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
-    double* W = work[wv];
+    RT* W = work[wv];
     const uint32_t n_groups = (n_slots + EW - 1) / EW;
     const uint32_t pa = lane < 60 ? lane / 3u : 19u, pi = lane < 60 ? lane - 3u * pa : 0u;
 
@@ -192,7 +209,7 @@ __global__ __launch_bounds__(256) void free_hex20_kernel(const double* __restric
     // loaded into registers while the current group is computed: lane value j
     // is entry idx = lane + 64 j of the group's EW x 60 block
     constexpr int NL = (EW * 60 + 63) / 64;
-    double px[NL], pu[NL], ps = 0.0;
+    RT px[NL], pu[NL], ps = RT(0);
     uint32_t pm[NL];
     auto issue = [&](uint32_t grp) {
         const uint32_t first = grp * EW, n_here = min((uint32_t)EW, n_slots - first);
@@ -203,22 +220,22 @@ __global__ __launch_bounds__(256) void free_hex20_kernel(const double* __restric
             const bool ok = idx < EW * 60u && e < n_here;
             // branch-free: entries outside the group load the group's first entry
             const uint64_t src = (slot0 + (ok ? e : 0u)) * 60u + (ok ? idx - 60u * e : 0u);
-            const double xv = xe[src];
+            const RT xv = (RT)xe[src];
             const uint32_t mv = DIAG ? 0u : map[src];
-            px[j] = ok ? xv : 0.0;
+            px[j] = ok ? xv : RT(0);
             pm[j] = ok ? mv : 0u;
         }
         const uint32_t e = lane / QP;
-        const double sv = scale[slot0 + min(e, n_here - 1u)];
-        ps = e < n_here ? sv : 0.0;
+        const RT sv = (RT)scale[slot0 + min(e, n_here - 1u)];
+        ps = e < n_here ? sv : RT(0);
     };
     auto gather = [&]() {
 #pragma unroll
         for (int j = 0; j < NL; j++) {
-            double v = 0.0;
+            RT v = RT(0);
             if constexpr (!DIAG) {
                 v = u[pm[j] & kMask];  // entries outside the group read u[0]; their elements are not stored
-                if (MASKED && (pm[j] & kFlag)) v = 0.0;
+                if (MASKED && (pm[j] & kFlag)) v = RT(0);
             }
             pu[j] = v;
         }
@@ -245,62 +262,62 @@ __global__ __launch_bounds__(256) void free_hex20_kernel(const double* __restric
                 W[e * kElemRow + 6u * a + 3u + k] = pu[j];
             }
         }
-        const double sc_here = ps;
+        const RT sc_here = ps;
         lds_sync_wave();
         if (next < n_groups) issue(next);
 
         // phase 1
         {
             const uint32_t e = lane / QP, q = lane % QP;
-            const double* XU = W + e * kElemRow;
-            const double* T = tab + q * kTabRow;
-            double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            const RT* XU = W + e * kElemRow;
+            const RT* T = tab + q * kTabRow;
+            RT J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll 4
             for (int a = 0; a < 20; a++) {
-                const double2 t01 = *reinterpret_cast<const double2*>(T + 4 * a);
-                const double dn[3] = {t01.x, t01.y, T[4 * a + 2]};
-                const double2 p0 = *reinterpret_cast<const double2*>(XU + 6 * a);
-                const double2 p1 = *reinterpret_cast<const double2*>(XU + 6 * a + 2);
-                const double xs[3] = {p0.x, p0.y, p1.x};
+                const RT2 t01 = *reinterpret_cast<const RT2*>(T + 4 * a);
+                const RT dn[3] = {t01.x, t01.y, T[4 * a + 2]};
+                const RT2 p0 = *reinterpret_cast<const RT2*>(XU + 6 * a);
+                const RT2 p1 = *reinterpret_cast<const RT2*>(XU + 6 * a + 2);
+                const RT xs[3] = {p0.x, p0.y, p1.x};
 #pragma unroll
                 for (int k = 0; k < 3; k++)
 #pragma unroll
                     for (int d = 0; d < 3; d++) J[k][d] += xs[k] * dn[d];
                 if constexpr (!DIAG) {
-                    const double2 p2 = *reinterpret_cast<const double2*>(XU + 6 * a + 4);
-                    const double us[3] = {p1.y, p2.x, p2.y};
+                    const RT2 p2 = *reinterpret_cast<const RT2*>(XU + 6 * a + 4);
+                    const RT us[3] = {p1.y, p2.x, p2.y};
 #pragma unroll
                     for (int k = 0; k < 3; k++)
 #pragma unroll
                         for (int d = 0; d < 3; d++) H[k][d] += us[k] * dn[d];
                 }
             }
-            double inv[3][3];
-            const double det = jac_inv_r(J, inv);
-            double w = e < n_here ? det * wq[q] * sc_here : 0.0;
+            RT inv[3][3];
+            const RT det = jac_inv_r(J, inv);
+            RT w = e < n_here ? det * wq[q] * sc_here : RT(0);
             // DIAG: a non-positive Jacobian at a real point marks the element (NaN diagonal)
-            if (DIAG && e < n_here && q < nq && !(det > 0.0)) w = __builtin_nan("");
-            double S[kH20Slot];
+            if (DIAG && e < n_here && q < nq && !(det > RT(0))) w = (RT)__builtin_nan("");
+            RT S[kH20Slot];
             if constexpr (DIAG) {
 #pragma unroll
                 for (int k = 0; k < 3; k++)
 #pragma unroll
                     for (int d = 0; d < 3; d++) S[4 * k + d] = inv[d][k];
             } else {
-                double g[3][3];  // g[i][k] = du_i/dx_k
+                RT g[3][3];  // g[i][k] = du_i/dx_k
 #pragma unroll
                 for (int i = 0; i < 3; i++)
 #pragma unroll
                     for (int k = 0; k < 3; k++)
                         g[i][k] = H[i][0] * inv[0][k] + H[i][1] * inv[1][k] + H[i][2] * inv[2][k];
-                const double exx = g[0][0] * w, eyy = g[1][1] * w, ezz = g[2][2] * w;
-                double sg[3][3];
-                sg[0][0] = c1 * exx + c2 * eyy + c2 * ezz;
-                sg[1][1] = c2 * exx + c1 * eyy + c2 * ezz;
-                sg[2][2] = c2 * exx + c2 * eyy + c1 * ezz;
-                sg[0][1] = sg[1][0] = (c3 * w) * (g[0][1] + g[1][0]);
-                sg[1][2] = sg[2][1] = (c3 * w) * (g[1][2] + g[2][1]);
-                sg[0][2] = sg[2][0] = (c3 * w) * (g[0][2] + g[2][0]);
+                const RT exx = g[0][0] * w, eyy = g[1][1] * w, ezz = g[2][2] * w;
+                RT sg[3][3];
+                sg[0][0] = k1 * exx + k2 * eyy + k2 * ezz;
+                sg[1][1] = k2 * exx + k1 * eyy + k2 * ezz;
+                sg[2][2] = k2 * exx + k2 * eyy + k1 * ezz;
+                sg[0][1] = sg[1][0] = (k3 * w) * (g[0][1] + g[1][0]);
+                sg[1][2] = sg[2][1] = (k3 * w) * (g[1][2] + g[2][1]);
+                sg[0][2] = sg[2][0] = (k3 * w) * (g[0][2] + g[2][0]);
 #pragma unroll
                 for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -308,37 +325,37 @@ __global__ __launch_bounds__(256) void free_hex20_kernel(const double* __restric
                         S[4 * i + d] = inv[d][0] * sg[i][0] + inv[d][1] * sg[i][1] + inv[d][2] * sg[i][2];
             }
             S[3] = w;
-            S[7] = S[11] = 0.0;
+            S[7] = S[11] = RT(0);
             lds_sync_wave();  // every lane's reads of the node values precede the slot writes that overlay them
-            double* slotp = W + lane * kH20Slot;
+            RT* slotp = W + lane * kH20Slot;
 #pragma unroll
-            for (int j = 0; j < kH20Slot; j += 2) *reinterpret_cast<double2*>(slotp + j) = make_double2(S[j], S[j + 1]);
+            for (int j = 0; j < kH20Slot; j += 2) *reinterpret_cast<RT2*>(slotp + j) = Vec2<RT>::make(S[j], S[j + 1]);
         }
         lds_sync_wave();
         if (next < n_groups) gather();
 
         // phase 2: f[e] = sum_q dN_q(a) . P_q(:, i) of element e
-        double f[EW];
+        RT f[EW];
 #pragma unroll
-        for (int e = 0; e < EW; e++) f[e] = 0.0;
-        auto point = [&](uint32_t e, uint32_t q, double d0, double d1, double d2) {
-            const double* S = W + (e * QP + q) * kH20Slot;
+        for (int e = 0; e < EW; e++) f[e] = RT(0);
+        auto point = [&](uint32_t e, uint32_t q, RT d0, RT d1, RT d2) {
+            const RT* S = W + (e * QP + q) * kH20Slot;
             if constexpr (DIAG) {
-                double g[3];
+                RT g[3];
 #pragma unroll
                 for (int k = 0; k < 3; k++) g[k] = d0 * S[4 * k] + d1 * S[4 * k + 1] + d2 * S[4 * k + 2];
-                const double gi = pi == 0 ? g[0] : (pi == 1 ? g[1] : g[2]);
-                const double gg = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
-                f[e] += (c1 * gi * gi + c3 * (gg - gi * gi)) * S[3];
+                const RT gi = pi == 0 ? g[0] : (pi == 1 ? g[1] : g[2]);
+                const RT gg = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+                f[e] += (k1 * gi * gi + k3 * (gg - gi * gi)) * S[3];
             } else {
-                const double2 p01 = *reinterpret_cast<const double2*>(S + 4 * pi);
-                const double p2 = S[4 * pi + 2];
+                const RT2 p01 = *reinterpret_cast<const RT2*>(S + 4 * pi);
+                const RT p2 = S[4 * pi + 2];
                 f[e] += d0 * p01.x + d1 * p01.y + d2 * p2;
             }
         };
         for (uint32_t q = 0; q < nq; q++) {
-            const double2 t01 = *reinterpret_cast<const double2*>(tab + q * kTabRow + 4 * pa);
-            const double t2 = tab[q * kTabRow + 4 * pa + 2];
+            const RT2 t01 = *reinterpret_cast<const RT2*>(tab + q * kTabRow + 4 * pa);
+            const RT t2 = tab[q * kTabRow + 4 * pa + 2];
 #pragma unroll
             for (uint32_t e = 0; e < EW; e++) point(e, q, t01.x, t01.y, t2);
         }
@@ -356,7 +373,7 @@ __global__ __launch_bounds__(256) void free_hex20_kernel(const double* __restric
     }
 }
 
-template <int QP>
+template <int QP, typename RT>
 void launch_hex20(bool masked, bool atomic, bool diag, const FreeArgs& a, hipStream_t s)
 {
     constexpr uint32_t per_block = 4u * (64u / QP);
@@ -370,35 +387,67 @@ void launch_hex20(bool masked, bool atomic, bool diag, const FreeArgs& a, hipStr
     if (res == 0) {
         int cus = 0, per_cu = 0;
         AMX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        AMX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, free_hex20_kernel<QP, false, false, false>,
+        AMX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, free_hex20_kernel<QP, RT, false, false, false>,
                                                              256, 0));
         res = std::max(1, cus * std::max(per_cu, 1));
         resident[dev].store(res, std::memory_order_relaxed);
     }
     const uint32_t need = (a.n_slots + per_block - 1) / per_block;
     const dim3 grid(std::min<uint32_t>(need, (uint32_t)res)), block(256);
+    const RT* u = static_cast<const RT*>(a.u);
+    RT* out = static_cast<RT*>(a.out);
 #define AMX_H20(M, A, D)                                                                                           \
-    free_hex20_kernel<QP, M, A, D><<<grid, block, 0, s>>>(a.xe, a.scale, a.map, a.u, a.out, a.ve_base, a.ve_rs,  \
-                                                          a.ve_ss, a.slot_begin, a.n_slots, a.qp, a.nq, a.c1, a.c2, \
-                                                          a.c3, a.skip)
-    if (diag) AMX_H20(false, false, true);
-    else if (masked) { if (atomic) AMX_H20(true, true, false); else AMX_H20(true, false, false); }
+    free_hex20_kernel<QP, RT, M, A, D><<<grid, block, 0, s>>>(a.xe, a.scale, a.map, u, out, a.ve_base, a.ve_rs,     \
+                                                              a.ve_ss, a.slot_begin, a.n_slots, a.qp, a.nq, a.c1,   \
+                                                              a.c2, a.c3, a.skip)
+    if constexpr (std::is_same<RT, double>::value) {
+        if (diag) {
+            AMX_H20(false, false, true);
+            return;
+        }
+    }
+    if (masked) { if (atomic) AMX_H20(true, true, false); else AMX_H20(true, false, false); }
     else { if (atomic) AMX_H20(false, true, false); else AMX_H20(false, false, false); }
 #undef AMX_H20
 }
 
-template <int NN>
+template <int NN, typename RT>
 void launch_laplace(bool masked, bool atomic, bool diag, const FreeArgs& a, hipStream_t s)
 {
     const dim3 grid((a.n_slots + 255) / 256), block(256);
+    const RT* u = static_cast<const RT*>(a.u);
+    RT* out = static_cast<RT*>(a.out);
 #define AMX_LAP(M, A, D)                                                                                           \
-    free_laplace_kernel<NN, M, A, D><<<grid, block, 0, s>>>(a.xe, a.scale, a.map, a.u, a.out, a.ve_base, a.ve_rs, \
-                                                            a.ve_ss, a.slot_begin, a.n_slots, a.stride, a.qp, a.nq, \
-                                                            a.skip)
-    if (diag) AMX_LAP(false, false, true);
-    else if (masked) { if (atomic) AMX_LAP(true, true, false); else AMX_LAP(true, false, false); }
+    free_laplace_kernel<NN, RT, M, A, D><<<grid, block, 0, s>>>(a.xe, a.scale, a.map, u, out, a.ve_base, a.ve_rs,  \
+                                                                a.ve_ss, a.slot_begin, a.n_slots, a.stride, a.qp,   \
+                                                                a.nq, a.skip)
+    if constexpr (std::is_same<RT, double>::value) {
+        if (diag) {
+            AMX_LAP(false, false, true);
+            return;
+        }
+    }
+    if (masked) { if (atomic) AMX_LAP(true, true, false); else AMX_LAP(true, false, false); }
     else { if (atomic) AMX_LAP(false, true, false); else AMX_LAP(false, false, false); }
 #undef AMX_LAP
+}
+
+template <typename RT>
+void launch_free_t(int kind, bool masked, bool atomic, bool diag, const FreeArgs& a, hipStream_t s)
+{
+    switch (kind) {
+    case AMX_ELEM_HEX8_LAPLACE: launch_laplace<8, RT>(masked, atomic, diag, a, s); break;
+    case AMX_ELEM_TET10_LAPLACE: launch_laplace<10, RT>(masked, atomic, diag, a, s); break;
+    case AMX_ELEM_HEX20_ELASTIC: {
+        AMX_REQUIRE(a.nq <= 64, AMX_NOT_IMPLEMENTED, "hex20: at most 64 quadrature points");
+        const uint32_t qp = free_padded_points(kind, a.nq);
+        if (qp == 8) launch_hex20<8, RT>(masked, atomic, diag, a, s);
+        else if (qp == 32) launch_hex20<32, RT>(masked, atomic, diag, a, s);
+        else launch_hex20<64, RT>(masked, atomic, diag, a, s);
+        break;
+    }
+    default: fail(AMX_INVALID_ARGUMENT, "unknown element kind " + std::to_string(kind));
+    }
 }
 
 }  // namespace
@@ -412,19 +461,9 @@ uint32_t free_padded_points(int kind, uint32_t nq)
 void launch_free(int kind, bool masked, bool atomic, bool diag, const FreeArgs& a, hipStream_t s)
 {
     if (a.n_slots == 0) return;
-    switch (kind) {
-    case AMX_ELEM_HEX8_LAPLACE: launch_laplace<8>(masked, atomic, diag, a, s); break;
-    case AMX_ELEM_TET10_LAPLACE: launch_laplace<10>(masked, atomic, diag, a, s); break;
-    case AMX_ELEM_HEX20_ELASTIC: {
-        AMX_REQUIRE(a.nq <= 64, AMX_NOT_IMPLEMENTED, "hex20: at most 64 quadrature points");
-        const uint32_t qp = free_padded_points(kind, a.nq);
-        if (qp == 8) launch_hex20<8>(masked, atomic, diag, a, s);
-        else if (qp == 32) launch_hex20<32>(masked, atomic, diag, a, s);
-        else launch_hex20<64>(masked, atomic, diag, a, s);
-        break;
-    }
-    default: fail(AMX_INVALID_ARGUMENT, "unknown element kind " + std::to_string(kind));
-    }
+    AMX_REQUIRE(!(diag && a.f32), AMX_INVALID_ARGUMENT, "the FREE diagonal is computed in FP64");
+    if (a.f32) launch_free_t<float>(kind, masked, atomic, diag, a, s);
+    else launch_free_t<double>(kind, masked, atomic, diag, a, s);
     AMX_HIP(hipGetLastError());
 }
 

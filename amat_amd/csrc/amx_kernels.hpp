@@ -101,8 +101,8 @@ struct FreeArgs {
     const double* xe;        // Laplace kinds: [3*nodes][stride]; hex20: [slot][60]
     const double* scale;     // [slot] stiffness factor (0 on padding slots)
     const uint32_t* map;     // as EbeArgs::map
-    const double* u;
-    double* out;             // ve (deterministic) or v (atomic)
+    const void* u;           // FP64, or FP32 with f32
+    void* out;               // ve (deterministic) or v (atomic), same type as u
     uint64_t ve_base;
     uint32_t ve_rs, ve_ss;   // as EbeArgs
     uint32_t slot_begin, n_slots, stride;
@@ -110,8 +110,9 @@ struct FreeArgs {
     uint32_t nq;             // quadrature points (hex20: table padded to free_padded_points rows)
     double c1, c2, c3;       // isotropic constitutive entries (hex20)
     const uint32_t* skip = nullptr;
+    bool f32 = false;        // compute and vectors in FP32 (inner solves); FP64 otherwise
 };
-// kind: amx_element_kind.  diag: write diag(s_e K_e) into ve rows instead.
+// kind: amx_element_kind.  diag: write diag(s_e K_e) into ve rows instead (FP64 only).
 void launch_free(int kind, bool masked, bool atomic, bool diag, const FreeArgs& a, hipStream_t s);
 uint32_t free_padded_points(int kind, uint32_t nq);
 

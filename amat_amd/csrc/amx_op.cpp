@@ -68,14 +68,15 @@ void upload_free(Op& op, Group& g, hipStream_t s)
 }
 
 FreeArgs free_args(const Op& op, const Group& g, const void* u, void* out, uint32_t slot_begin, uint32_t n_slots,
-                   const uint32_t* skip)
+                   const uint32_t* skip, bool f32 = false)
 {
     FreeArgs a;
     a.xe = g.xe.as<double>();
     a.scale = g.fscale.as<double>();
     a.map = g.map.as<uint32_t>();
-    a.u = static_cast<const double*>(u);
-    a.out = static_cast<double*>(out);
+    a.u = u;
+    a.out = out;
+    a.f32 = f32;
     a.ve_base = g.ve_base;
     a.ve_rs = g.ve_rs;
     a.ve_ss = g.ve_ss;
@@ -107,6 +108,7 @@ void finalize_op(Op& op)
     const bool free_m = op.is_free();
     if (free_m) {
         AMX_REQUIRE(op.elem_kind != 0, AMX_INVALID_STATE, "method 2 (FREE) needs set_element_function before finalize");
+        AMX_REQUIRE(!op.has(AMX_PREC_BF16), AMX_NOT_IMPLEMENTED, "method 2 (FREE) has no BF16 form");
         uint32_t nq = 0;
         const std::vector<double> tab =
             free_quadrature(op.elem_kind, op.elem_ngt, nq, free_padded_points(op.elem_kind, op.elem_nq));
@@ -375,13 +377,16 @@ void ebe_launch(Op& op, KType kt, bool vt_double, bool masked, int phase, const 
     if (!ve) ve = op.ve.p;
     const bool atomic = op.scatter == AMX_SCATTER_ATOMIC;
     if (op.is_free()) {
-        AMX_REQUIRE(kt == KType::F64 && vt_double, AMX_NOT_IMPLEMENTED, "method 2 (FREE) runs in FP64 only");
+        // FP64, or FP32 arithmetic and vectors (the inner solves of the
+        // mixed-precision solvers; nothing is stored per precision)
+        AMX_REQUIRE((kt == KType::F64 && vt_double) || (kt == KType::F32 && !vt_double), AMX_NOT_IMPLEMENTED,
+                    "method 2 (FREE) runs in FP64 or FP32");
         for (auto& g : op.groups) {
             uint32_t t0 = 0, nt = g.n_tiles;
             if (phase == 0) nt = g.tiles_phase0;
             else if (phase == 1) { t0 = g.tiles_phase0; nt = g.n_tiles - t0; }
             launch_free(op.elem_kind, masked, atomic, false,
-                        free_args(op, g, u, atomic ? v : ve, t0 * g.G, nt * g.G, skip), s);
+                        free_args(op, g, u, atomic ? v : ve, t0 * g.G, nt * g.G, skip, !vt_double), s);
         }
         return;
     }

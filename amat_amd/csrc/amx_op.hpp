@@ -103,8 +103,9 @@ struct Op {
     double penalty() const { return penalty_set ? penalty_value : kPenaltyFactor * trace; }
     // the caller's stream; NULL is the HIP default stream (not the handle's)
     hipStream_t pick(void* s) const { return static_cast<hipStream_t>(s); }
-    bool has(int prec_bit) const { return (prec_mask & prec_bit) != 0; }
     bool is_free() const { return matfree_type == AMX_MATFREE_FREE; }
+    // FREE operators apply in FP32 with nothing stored: FP32 is always available
+    bool has(int prec_bit) const { return (prec_mask & prec_bit) != 0 || (prec_bit == AMX_PREC_FP32 && is_free()); }
 };
 
 // nodes of a FREE element kind (amx_element_kind)

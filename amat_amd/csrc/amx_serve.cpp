@@ -10,7 +10,7 @@
 // The host looks at the jobs' scalars once per step (one device-to-host copy);
 // jobs leave when done and new ones are admitted between steps.
 //
-// Jobs on method-2 (FREE) operators (FP64 CG) launch their in-kernel element
+// Jobs on method-2 (FREE) operators (FP64 CG, or FP32 inner solves) launch their in-kernel element
 // kernels one job at a time; the gather and the CG phases stay batched.
 //
 // Every job's arithmetic depends on its own data only: the scatter is the
@@ -224,9 +224,9 @@ void launch_iteration(amx_server& S)
         amx::launch_ebe_batched(n, kt, f64, t.dev_jobs.as<amx::EbeJob>(), t.dev_prefix.as<uint32_t>(), t.n, t.total,
                                 s);
     }
-    for (Job* jp : S.free_jobs) {  // FP64 element kernels from coordinates, into the job's element rows
+    for (Job* jp : S.free_jobs) {  // element kernels from coordinates (FP64 or FP32), into the job's element rows
         Op& op = jp->h->op;
-        amx::ebe_launch(op, KType::F64, true, op.bc_operator == AMX_BCOP_IDENTITY, 2, jp->p.p, nullptr, s,
+        amx::ebe_launch(op, jp->kt, jp->f64iter, op.bc_operator == AMX_BCOP_IDENTITY, 2, jp->p.p, nullptr, s,
                         &S.it_sc(*jp)->done, jp->ve.p);
     }
     for (int f = 0; f < 2; f++) {
@@ -281,6 +281,19 @@ void replace_residual(amx_server& S, Job& j)
     amx::ru_replace(false, false, j.b.as<double>(), j.q64.as<double>(), j.r.p, j.z.p, j.p.p, j.dinv.as<double>(), j.n,
                     j.partials.as<double>(), S.it_sc(j), s);
     amx::cg_p(false, j.p.p, j.z.p, j.n, S.it_sc(j), s);
+}
+
+// FP64 CG: the stopped recurrence's residual replaced by the true b - A x
+// (the job ends when that meets rtol or a limit, else it iterates on), as
+// amx_cg_solve_dev does
+void replace_residual64(amx_server& S, Job& j)
+{
+    hipStream_t s = S.stream;
+    Op& op = j.h->op;
+    amx::apply(op, KType::F64, true, op.bc_operator, j.x.p, j.q.p, s, nullptr, j.ve.p);
+    amx::ru_replace(true, false, j.b.as<double>(), j.q.as<double>(), j.r.p, j.z.p, j.p.p, j.dinv.as<double>(), j.n,
+                    j.partials.as<double>(), S.it_sc(j), s);
+    amx::cg_p(true, j.p.p, j.z.p, j.n, S.it_sc(j), s);
 }
 
 // true residual ||b - A x|| of a reliable-update job stopped by a breakdown
@@ -341,8 +354,17 @@ void advance(amx_server& S)
                 }
             } else if (j.method == AMX_PREC_FP64) {
                 j.cg_iters = (int)it.iter;
-                final_residual(S, j);
-                j.phase = kFinalPending;
+                if (it.breakdown) {
+                    final_residual(S, j);
+                    j.phase = kFinalPending;
+                } else if (it.state == 2) {  // decided on the true residual
+                    j.outer = (int)it.n_upd - 1;
+                    j.rel = it.bb > 0 ? std::sqrt(it.rr / it.bb) : 0.0;
+                    j.converged = j.rel <= j.rtol * 1.0000001 || it.bb == 0.0;
+                    finish(S, j);
+                } else {
+                    replace_residual64(S, j);  // the job runs on in the next step unless final
+                }
             } else {
                 j.inner_total += (int)it.iter;
                 j.stalled = it.iter == 0;
@@ -438,8 +460,8 @@ int amx_server_submit(amx_server* S, amx_op* h, int method, const double* b_host
                     "inner_rtol must be in (0, 1)");
         Op& op = h->op;
         AMX_REQUIRE(op.finalized, AMX_INVALID_STATE, "operator not finalized");
-        AMX_REQUIRE(!op.is_free() || method == AMX_PREC_FP64, AMX_NOT_IMPLEMENTED,
-                    "method-2 (FREE) operators are served with FP64 CG");
+        AMX_REQUIRE(!op.is_free() || base != AMX_PREC_BF16, AMX_NOT_IMPLEMENTED,
+                    "method-2 (FREE) operators are served in FP64 or FP32");
         AMX_REQUIRE(op.size == 1, AMX_NOT_IMPLEMENTED, "served solves are one-rank");
         AMX_REQUIRE(op.device == S->device, AMX_INVALID_ARGUMENT, "operator on another device");
         AMX_REQUIRE(op.scatter == AMX_SCATTER_DETERMINISTIC, AMX_NOT_IMPLEMENTED,
@@ -499,7 +521,7 @@ int amx_server_submit(amx_server* S, amx_op* h, int method, const double* b_host
                             j.dinv.as<double>(), j.n, j.partials.as<double>(), S->it_sc(j), s);
             j.phase = kIter;
         } else if (j.f64iter) {
-            amx::cg_reset(S->it_sc(j), rtol, (uint32_t)max_it, s);
+            amx::cg_reset(S->it_sc(j), rtol, (uint32_t)max_it, s, 0.0, amx::kCgReplacements);
             amx::cg_init(true, j.b.p, j.dinv.as<double>(), j.x.p, j.r.p, j.z.p, j.p.p, j.n, j.partials.as<double>(),
                          S->it_sc(j), s);
             j.phase = kIter;

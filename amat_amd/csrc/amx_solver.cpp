@@ -82,14 +82,13 @@ void build_dinv_ws(Op& op, hipStream_t s) { amx::build_dinv(op, op.ws_dinv.as<do
 
 // PCG on A x = b (x starts at 0), vectors of type f64 ? double : float, the
 // operator with storage kt and the handle's BC form.  Returns iterations.
-CgScalars pcg(Op& op, KType kt, bool f64, const void* b, void* x, void* r, void* z, void* p, void* q, double rtol,
-              int max_it, CgScalars* dsc, hipStream_t s)
+// PCG iterations from the current state until the device stops them.
+CgScalars cg_loop(Op& op, KType kt, bool f64, void* x, void* r, void* z, void* p, void* q, int max_it,
+                  CgScalars* dsc, hipStream_t s)
 {
     const uint32_t n = op.n_total;
     double* partials = op.ws_partials.as<double>();
     const double* dinv = op.ws_dinv.as<double>();
-    set_scalars(op, dsc, rtol, max_it, s);
-    cg_init(f64, b, dinv, x, r, z, p, n, partials, dsc, s);
     // poll every `chunk` iterations: about 1/8 of the iterations of a small
     // solve, every other one of a large solve
     const uint32_t chunk = n > (1u << 21) ? 2 : (n > (1u << 17) ? 4 : 8);
@@ -105,6 +104,14 @@ CgScalars pcg(Op& op, KType kt, bool f64, const void* b, void* x, void* r, void*
         const CgScalars sc = read_scalars(op, dsc, s);
         if (sc.done || launched >= max_it) return sc;
     }
+}
+
+CgScalars pcg(Op& op, KType kt, bool f64, const void* b, void* x, void* r, void* z, void* p, void* q, double rtol,
+              int max_it, CgScalars* dsc, hipStream_t s, int max_upd = 0)
+{
+    set_scalars(op, dsc, rtol, max_it, s, 0.0, max_upd);
+    cg_init(f64, b, op.ws_dinv.as<double>(), x, r, z, p, op.n_total, op.ws_partials.as<double>(), dsc, s);
+    return cg_loop(op, kt, f64, x, r, z, p, q, max_it, dsc, s);
 }
 
 // Mixed-precision PCG with reliable updates (residual replacement: Sleijpen
@@ -189,22 +196,32 @@ int amx_cg_solve_dev(amx_op* h, const double* b, double* x, double rtol, int max
         amx::ensure_ws(op);
         amx::build_dinv_ws(op, s);
         amx::CgScalars* dsc = op.ws_sc.as<amx::CgScalars>();
-        const amx::CgScalars sc = amx::pcg(op, KType::F64, true, b, x, op.ws_r.p, op.ws_z.p, op.ws_p.p, op.ws_q.p,
-                                           rtol, max_it, dsc, s);
-        // true residual ||b - A x|| / ||b|| in FP64
-        amx::apply(op, KType::F64, true, op.bc_operator, x, op.ws_q.p, s);
-        amx::residual(b, op.ws_q.as<double>(), op.ws_r.as<double>(), op.n_total, op.ws_partials.as<double>(), dsc, s);
-        const amx::CgScalars fin = amx::read_scalars(op, dsc, s);
+        amx::CgScalars sc = amx::pcg(op, KType::F64, true, b, x, op.ws_r.p, op.ws_z.p, op.ws_p.p, op.ws_q.p, rtol,
+                                     max_it, dsc, s, amx::kCgReplacements);
+        // the recurrence stopped: its residual is replaced by the true b - A x
+        // (FP64); the solve ends there when that meets rtol (or a limit is
+        // reached), else CG continues the direction from the true residual
+        amx::CgScalars fin;
+        for (;;) {
+            amx::apply(op, KType::F64, true, op.bc_operator, x, op.ws_q.p, s);
+            amx::ru_replace(true, false, b, op.ws_q.as<double>(), op.ws_r.p, op.ws_z.p, op.ws_p.p,
+                            op.ws_dinv.as<double>(), op.n_total, op.ws_partials.as<double>(), dsc, s);
+            amx::cg_p(true, op.ws_p.p, op.ws_z.p, op.n_total, dsc, s);  // a no-op once final
+            fin = amx::read_scalars(op, dsc, s);
+            if (fin.done || fin.breakdown) break;
+            sc = amx::cg_loop(op, KType::F64, true, x, op.ws_r.p, op.ws_z.p, op.ws_p.p, op.ws_q.p, max_it, dsc, s);
+            if (sc.breakdown) break;
+        }
         const double rel = fin.bb > 0 ? std::sqrt(fin.rr / fin.bb) : 0.0;
         if (stats) {
-            stats->iterations = (int32_t)sc.iter;
-            stats->outer_iterations = 0;
+            stats->iterations = (int32_t)fin.iter;
+            stats->outer_iterations = (int32_t)fin.n_upd - 1;
             stats->converged = rel <= rtol * 1.0000001 || fin.bb == 0.0;
             stats->precision = AMX_PREC_FP64;
             stats->rel_residual = rel;
             stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         }
-        AMX_REQUIRE(!sc.breakdown, AMX_NOT_CONVERGED, "CG breakdown: p^T A p <= 0 (operator not SPD)");
+        AMX_REQUIRE(!fin.breakdown, AMX_NOT_CONVERGED, "CG breakdown: p^T A p <= 0 (operator not SPD)");
     });
 }
 

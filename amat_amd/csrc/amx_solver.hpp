@@ -39,6 +39,9 @@ struct CgJob {
 // counters live in a per-solve buffer of kPartialsBytes, zeroed when allocated.
 constexpr uint32_t kMaxRedBlocks = 1024;
 constexpr size_t kPartialsBytes = 4096 * sizeof(double);
+// FP64 CG: residual replacements allowed after the recurrence stops (the
+// first one measures the true residual; more run only when it misses rtol)
+constexpr uint32_t kCgReplacements = 4;
 uint32_t reduction_grid(uint32_t n);
 void cg_init(bool f64, const void* b, const double* dinv, void* x, void* r, void* z, void* p, uint32_t n,
              double* partials, CgScalars* sc, hipStream_t s);

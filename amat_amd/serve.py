@@ -146,7 +146,8 @@ def rhs_for(p: problems.Problem, seed: int) -> np.ndarray:
 
 def build_operator(p: problems.Problem, precision=PREC.FP64 | PREC.FP32 | PREC.BF16, free: bool = False) -> aMatFree:
     """The served operator: stored element matrices in the given storage
-    precisions, or (free=True) method 2, K_e recomputed in-kernel (FP64 CG only)."""
+    precisions, or (free=True) method 2, K_e recomputed in-kernel (FP64 CG, or FP32
+    inner solves with FP64 reliable updates)."""
     if free:
         return p.build_free_operator(aMatFree, Maps, bcop=BCOP.IDENTITY)
     return p.build_operator(aMatFree, Maps, precision=precision, bcop=BCOP.IDENTITY)

@@ -184,12 +184,12 @@ def free_mode(args) -> dict:
     operator; Gauss rules 2^3 (the reference example's NGT) and 3^3."""
     import torch
     from amat_amd import BCOP, PREC, Maps, aMatFree, problems
-    res = {"metric": "method 2 (FREE) FP64 matvec vs stored K_e", "n": args.n}
+    res = {"metric": "method 2 (FREE) matvec vs stored K_e (FP64; FP32 forms for hex20)", "n": args.n}
     s = torch.cuda.current_stream()
     for ngt in (3, 2):
         t0 = time.time()
         p = problems.elasticity_hex20(args.n, ngt=ngt)
-        st = p.build_operator(aMatFree, Maps, precision=PREC.FP64, bcop=BCOP.PENALTY_DIAG, penalty=1e8)
+        st = p.build_operator(aMatFree, Maps, precision=PREC.FP64 | PREC.FP32, bcop=BCOP.PENALTY_DIAG, penalty=1e8)
         fr = p.build_free_operator(aMatFree, Maps, bcop=BCOP.PENALTY_DIAG, penalty=1e8)
         log(f"[free] hex20 {args.n}^3 ngt {ngt}: {p.n_elem} elements, {p.n_dofs} DoFs, operators built "
             f"({time.time() - t0:.1f}s)")
@@ -220,7 +220,23 @@ def free_mode(args) -> dict:
                 + (f", {inf.flops / (ms_e * 1e-3) / 1e12:.2f} TFLOP/s, rel L2 vs stored {rel:.2e}"
                    if name == "free" else ""))
         res[f"ngt{ngt}_speedup"] = res[f"ngt{ngt}_stored"]["ms"] / res[f"ngt{ngt}_free"]["ms"]
-        del st, fr, u, v0, v1
+        # FP32 forms (the inner operator of the mixed-precision solvers): FP32
+        # element matrices vs the FREE kernels in FP32 arithmetic
+        u32 = u.float()
+        v32 = torch.empty_like(u32)
+        fr.matmult_f32_dev(PREC.FP32, v32, u32, s)
+        fr.matmult_dev(v1, u, s)
+        torch.cuda.synchronize()
+        rel32 = float(torch.linalg.norm(v32.double() - v1) / torch.linalg.norm(v1))
+        for name, op in (("stored_fp32", st), ("free_fp32", fr)):
+            ms = time_matvec(lambda a, b_, st_, o=op: o.matmult_f32_dev(PREC.FP32, a, b_, st_), v32, u32, args.steps,
+                             args.warmup, torch)
+            res[f"ngt{ngt}_{name}"] = {"ms": ms, "GDoF/s": N / ms / 1e6}
+            if name == "free_fp32":
+                res[f"ngt{ngt}_{name}"]["rel_l2_vs_free_fp64"] = rel32
+            log(f"[free] ngt {ngt} {name}: {ms:.4f} ms/matvec ({N / ms / 1e6:.2f} GDoF/s)"
+                + (f", rel L2 vs the FP64 form {rel32:.2e}" if name == "free_fp32" else ""))
+        del st, fr, u, v0, v1, u32, v32
         torch.cuda.empty_cache()
     # Laplace kinds, one lane per element: hex8 Poisson (2x2x2 Gauss) at (2n)^3,
     # jittered tet10 Poisson (8-point rule) at n^3 x 6
@@ -345,15 +361,18 @@ def serve_mode(args) -> dict:
     rates = [float(r) for r in args.serve_rates.split(",")]
     # fp32 / bf16: refinement restarted per outer step (inner tolerance --inner-rtol);
     # fp32r / bf16r: reliable updates (replacement threshold --ru-delta)
-    # fp64free: FP64 CG on method-2 operators (K_e recomputed in-kernel; own cache)
+    # fp64free: FP64 CG on method-2 operators (K_e recomputed in-kernel; own cache);
+    # fp32rfree: the same operators, FP32 inner solves with FP64 reliable updates
     methods = {"fp64": PREC.FP64, "fp32": PREC.FP32, "bf16": PREC.BF16,
-               "fp32r": int(PREC.FP32) | IR_RELIABLE, "bf16r": int(PREC.BF16) | IR_RELIABLE, "fp64free": PREC.FP64}
+               "fp32r": int(PREC.FP32) | IR_RELIABLE, "bf16r": int(PREC.BF16) | IR_RELIABLE, "fp64free": PREC.FP64,
+               "fp32rfree": int(PREC.FP32) | IR_RELIABLE}
+    reliable = {"fp32r", "bf16r", "fp32rfree"}
     names = args.serve_methods.split(",")
     streams = {r: serve.make_requests(args.serve_requests, r, seed=args.serve_seed) for r in rates}
     keys = sorted({(q.kind, q.n) for st in streams.values() for q in st})
     budget = int(args.serve_budget_gb * 2**30)
     caches = {}
-    for free in sorted({n == "fp64free" for n in names}):
+    for free in sorted({n.endswith("free") for n in names}):
         cache = serve.OperatorCache(free=free)
         t0 = time.time()
         for i, key in enumerate(keys):
@@ -373,8 +392,8 @@ def serve_mode(args) -> dict:
     for rate in rates:
         for name in names:
             server = serve.Server(max_jobs=args.serve_slots)
-            loop = serve.ServingLoop(server, caches[name == "fp64free"], methods[name], budget, n_iter=args.serve_iters,
-                                     inner_rtol=args.ru_delta if name.endswith("r") else args.inner_rtol)
+            loop = serve.ServingLoop(server, caches[name.endswith("free")], methods[name], budget, n_iter=args.serve_iters,
+                                     inner_rtol=args.ru_delta if name in reliable else args.inner_rtol)
             last = [0.0]
 
             def prog(el, fin, run, pend):

@@ -227,8 +227,10 @@ int amx_op_set_matfree_type(amx_op* op, int matfree_type);
  * isotropic material (hex20); ngt: Gauss points per direction, as the
  * examples' NGT (hex20: ngt^3 <= 64).  Every element must have the kind's DoF
  * count.  Sets the matfree type to FREE; finalize() builds no element
- * matrices.  FP64 only: the mixed-precision, multi-RHS and server entry
- * points reject a FREE operator. */
+ * matrices.  FP64, and an FP32 form (FP32 arithmetic and vectors, nothing
+ * stored) for amx_op_matmult_f32_dev(op, AMX_PREC_FP32, ...), the
+ * mixed-precision solvers and served FP32 jobs; no BF16 form; the multi-RHS
+ * entry points reject a FREE operator. */
 int amx_op_set_element_function(amx_op* op, int kind, const double* xe, const double* scale,
                                 double E, double nu, uint32_t ngt);
 /* aMatFree::set_num_streams (include/aMatFree.hpp:230-237): number of HIP
@@ -333,14 +335,16 @@ int amx_op_synchronize(amx_op* op);
  * length n_total (one rank).  Stop: ||b - A x||_2 <= rtol ||b||_2. */
 typedef struct amx_solve_stats {
     int32_t iterations;         /* CG iterations (inner ones for IR) */
-    int32_t outer_iterations;   /* refinement steps (IR only) */
+    int32_t outer_iterations;   /* refinement steps (IR); FP64 CG: extra residual replacements */
     int32_t converged;
     int32_t precision;          /* AMX_PREC_* of the inner operator */
     double rel_residual;        /* true ||b - A x|| / ||b|| in FP64 at exit */
     double seconds;
 } amx_solve_stats;
 
-/* FP64 Jacobi-PCG. */
+/* FP64 Jacobi-PCG.  When the recurrence stops, its residual is replaced by
+ * the true b - A x; convergence is decided on that, and CG continues the
+ * direction from it while it misses rtol (at most 4 replacements). */
 int amx_cg_solve_dev(amx_op* op, const double* b, double* x, double rtol, int max_it,
                      amx_solve_stats* stats, void* stream);
 /* FP64 iterative refinement: true residual, alpha/beta, inner products and the

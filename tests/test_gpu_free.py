@@ -256,7 +256,57 @@ def test_free_operators_served(torch_gpu):
         np.testing.assert_array_equal(x, x1)
         assert st.iterations == it1
     with pytest.raises(AmatError):
-        srv.submit(ops[0], PREC.FP32, rhs[0])
+        srv.submit(ops[0], PREC.BF16, rhs[0])   # FREE has no BF16 form
+
+
+@pytest.mark.parametrize("which", list(CASES))
+def test_free_fp32_form(torch_gpu, which):
+    """The FP32 form of the FREE kernels (FP32 arithmetic and vectors, the
+    inner operator of the mixed-precision solvers) against the FP64 form:
+    within FP32 rounding of the element algebra."""
+    torch = torch_gpu
+    p = CASES[which]()
+    op = p.build_free_operator(aMatFree, Maps, bcop=BCOP.IDENTITY)
+    u = p.random_u()
+    v64 = run(torch, op.matmult_dev, u)
+    du = torch.from_numpy(u).float().cuda()
+    dv = torch.zeros_like(du)
+    op.matmult_f32_dev(PREC.FP32, dv, du)
+    torch.cuda.synchronize()
+    err = rel_l2(dv.cpu().numpy().astype(np.float64), v64)
+    assert 1e-9 < err <= 2e-5, err   # genuinely FP32, and as accurate as FP32 storage
+
+
+def test_free_fp32_reliable_updates(torch_gpu):
+    """FP32 FREE inner solves with FP64 reliable updates reach 1e-10 on hex20
+    elasticity, single solve and served (bit-identical alone and in a mix)."""
+    torch = torch_gpu
+    from amat_amd import serve
+    from amat_amd.amat import IR_RELIABLE
+    p = serve.build_problem("hex20", 4)
+    op = serve.build_operator(p, free=True)
+    b = serve.rhs_for(p, 3)
+    db = torch.from_numpy(b).cuda()
+    x64 = torch.zeros_like(db)
+    st64 = op.cg(db, x64, rtol=1e-10, max_it=20000)
+    method = int(PREC.FP32) | IR_RELIABLE
+    x = torch.zeros_like(db)
+    st = op.ir_cg(method, db, x, rtol=1e-10, inner_rtol=0.1, max_outer=200, max_inner=20000)
+    assert st.converged and st.rel_residual <= 1e-10
+    assert st.iterations <= 1.5 * st64.iterations
+    assert rel_l2(x.cpu().numpy(), x64.cpu().numpy()) <= 1e-8
+    srv = Server(max_jobs=4)
+    j1 = srv.submit(op, method, b, inner_rtol=0.1, max_outer=200)
+    other = serve.build_problem("tet10", 3)
+    j2 = srv.submit(serve.build_operator(other, free=True), PREC.FP64, serve.rhs_for(other, 5))
+    out = {}
+    while len(out) < 2:
+        srv.step(8)
+        for j in srv.poll():
+            out[j] = srv.result(j, p.n_dofs if j == j1 else other.n_dofs)
+    xs, sts = out[j1]
+    assert sts.converged and sts.iterations == st.iterations
+    np.testing.assert_array_equal(xs, x.cpu().numpy())
 
 
 def test_free_rejections(torch_gpu):
@@ -265,11 +315,11 @@ def test_free_rejections(torch_gpu):
     op = p.build_free_operator(aMatFree, Maps)
     x = torch.zeros(p.n_dofs, dtype=torch.float32, device="cuda")
     with pytest.raises(AmatError):
-        op.matmult_f32_dev(PREC.FP32, x, x)
+        op.matmult_f32_dev(PREC.BF16, x, x)      # no BF16 form
     with pytest.raises(AmatError):
         MultiRHS([op], PREC.FP32)
     with pytest.raises(AmatError):
-        Server(max_jobs=2).submit(op, PREC.FP32, np.zeros(p.n_dofs))   # FREE is served with FP64 CG only
+        Server(max_jobs=2).submit(op, PREC.BF16, np.zeros(p.n_dofs))
     # wrong element kind for the DoF count; FREE finalize without coordinates
     o2 = aMatFree(p.build_maps(Maps))
     with pytest.raises(AmatError):

@@ -117,3 +117,37 @@ def test_reliable_update_cg(torch_gpu, prec, which):
     # the FP64 solution against scipy on the assembled system
     xd = spla.spsolve(direct(p, bcop, pen), b)
     assert rel_l2(xr.cpu().numpy(), xd) < 1e-7
+
+
+def test_fp64_cg_residual_replacement(torch_gpu):
+    """FP64 CG replaces the stopped recurrence's residual by the true b - A x
+    and iterates on while that misses rtol: at a tolerance below what the
+    recurrence can certify, replacements run (up to the limit), the reported
+    residual is the true one, and the served job takes the same path
+    bit for bit."""
+    torch = torch_gpu
+    from amat_amd import serve
+    from amat_amd.serve import Server
+    p = serve.build_problem("hex20", 3)
+    op = serve.build_operator(p)
+    b = serve.rhs_for(p, 11)
+    db = torch.from_numpy(b).cuda()
+    x = torch.zeros_like(db)
+    st = op.cg(db, x, rtol=1e-16, max_it=20000)
+    assert st.outer_iterations >= 1          # the first true residual missed rtol
+    assert not st.converged and 1e-16 < st.rel_residual < 1e-12
+    xs = x.cpu().numpy()
+    srv = Server(max_jobs=2)
+    j = srv.submit(op, PREC.FP64, b, rtol=1e-16)
+    out = {}
+    while not out:
+        srv.step(8)
+        for k in srv.poll():
+            out[k] = srv.result(k, p.n_dofs)
+    x2, st2 = out[j]
+    np.testing.assert_array_equal(x2, xs)
+    assert (st2.iterations, st2.outer_iterations, st2.converged) == (st.iterations, st.outer_iterations, st.converged)
+    # at an attainable tolerance the first replacement confirms convergence
+    x.zero_()
+    st3 = op.cg(db, x, rtol=1e-10, max_it=20000)
+    assert st3.converged and st3.outer_iterations == 0 and st3.rel_residual <= 1e-10
