@@ -169,12 +169,18 @@ __global__ __launch_bounds__(256) void free_laplace_kernel(const double* __restr
 // u_x u_y u_z] at kElemRow doubles per element (likewise); the P slots
 // overlay the node values once phase 1 has read them.
 constexpr int kH20Slot = 12;   // S[4i+d] = P_di (DIAG: S[4k+d] = inv[d][k]), S[3] = w
-constexpr int kTabRow = 82;    // 80 used
-constexpr int kElemRow = 122;  // 120 used
+// LDS strides per compute type: FP64 reads two values per 16-byte access
+// (dN rows of 82, node values x y z u_x u_y u_z at 6 per node, 122 per
+// element); FP32 reads a whole dN row quadruple, a P row or a node's
+// x y z u_x in one 16-byte access (rows of 84, 8 values per node: x y z u_x
+// u_y u_z and 2 pad, 164 per element)
+template <typename RT> struct H20Lds;
+template <> struct H20Lds<double> { static constexpr int tab_row = 82, elem_row = 122, node = 6; };
+template <> struct H20Lds<float> { static constexpr int tab_row = 84, elem_row = 164, node = 8; };
 
-template <int QP> constexpr int h20_wave_doubles()
+template <int QP, typename RT> constexpr int h20_wave_values()
 {
-    return (64 / QP) * kElemRow > 64 * kH20Slot ? (64 / QP) * kElemRow : 64 * kH20Slot;
+    return (64 / QP) * H20Lds<RT>::elem_row > 64 * kH20Slot ? (64 / QP) * H20Lds<RT>::elem_row : 64 * kH20Slot;
 }
 
 template <int QP, typename RT, bool MASKED, bool ATOMIC, bool DIAG>
@@ -187,7 +193,9 @@ __global__ __launch_bounds__(256) void free_hex20_kernel(const double* __restric
                                                         const uint32_t* __restrict__ skip)
 {
     using RT2 = typename Vec2<RT>::type;
-    constexpr int EW = 64 / QP, WD = h20_wave_doubles<QP>();
+    constexpr int EW = 64 / QP, WD = h20_wave_values<QP, RT>();
+    constexpr int kTabRow = H20Lds<RT>::tab_row, kElemRow = H20Lds<RT>::elem_row, NS = H20Lds<RT>::node;
+    constexpr bool F32 = std::is_same<RT, float>::value;
     __shared__ __align__(16) RT tab[QP * kTabRow];
     __shared__ RT wq[QP];
     __shared__ __align__(16) RT work[4][WD];
@@ -252,14 +260,14 @@ __global__ __launch_bounds__(256) void free_hex20_kernel(const double* __restric
         const uint64_t slot0 = (uint64_t)slot_begin + first;
         const uint32_t next = grp + gridDim.x * 4u;
 
-        // element e's node a at W[e*kElemRow + 6a]: x y z, then u
+        // element e's node a at W[e*kElemRow + NS a]: x y z, then u
 #pragma unroll
         for (int j = 0; j < NL; j++) {
             const uint32_t idx = lane + 64u * j;
             if (idx < EW * 60u) {
                 const uint32_t e = idx / 60u, r = idx - e * 60u, a = r / 3u, k = r - 3u * a;
-                W[e * kElemRow + 6u * a + k] = px[j];
-                W[e * kElemRow + 6u * a + 3u + k] = pu[j];
+                W[e * kElemRow + NS * a + k] = px[j];
+                W[e * kElemRow + NS * a + 3u + k] = pu[j];
             }
         }
         const RT sc_here = ps;
@@ -274,18 +282,34 @@ __global__ __launch_bounds__(256) void free_hex20_kernel(const double* __restric
             RT J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll 4
             for (int a = 0; a < 20; a++) {
-                const RT2 t01 = *reinterpret_cast<const RT2*>(T + 4 * a);
-                const RT dn[3] = {t01.x, t01.y, T[4 * a + 2]};
-                const RT2 p0 = *reinterpret_cast<const RT2*>(XU + 6 * a);
-                const RT2 p1 = *reinterpret_cast<const RT2*>(XU + 6 * a + 2);
-                const RT xs[3] = {p0.x, p0.y, p1.x};
+                RT dn[3], xs[3], us[3];
+                if constexpr (F32) {
+                    const float4 t4 = *reinterpret_cast<const float4*>(T + 4 * a);
+                    const float4 x4 = *reinterpret_cast<const float4*>(XU + 8 * a);
+                    dn[0] = t4.x, dn[1] = t4.y, dn[2] = t4.z;
+                    xs[0] = x4.x, xs[1] = x4.y, xs[2] = x4.z;
+                    us[0] = x4.w;
+                    if constexpr (!DIAG) {
+                        const float2 u2 = *reinterpret_cast<const float2*>(XU + 8 * a + 4);
+                        us[1] = u2.x, us[2] = u2.y;
+                    }
+                } else {
+                    const RT2 t01 = *reinterpret_cast<const RT2*>(T + 4 * a);
+                    const RT2 p0 = *reinterpret_cast<const RT2*>(XU + 6 * a);
+                    const RT2 p1 = *reinterpret_cast<const RT2*>(XU + 6 * a + 2);
+                    dn[0] = t01.x, dn[1] = t01.y, dn[2] = T[4 * a + 2];
+                    xs[0] = p0.x, xs[1] = p0.y, xs[2] = p1.x;
+                    us[0] = p1.y;
+                    if constexpr (!DIAG) {
+                        const RT2 p2 = *reinterpret_cast<const RT2*>(XU + 6 * a + 4);
+                        us[1] = p2.x, us[2] = p2.y;
+                    }
+                }
 #pragma unroll
                 for (int k = 0; k < 3; k++)
 #pragma unroll
                     for (int d = 0; d < 3; d++) J[k][d] += xs[k] * dn[d];
                 if constexpr (!DIAG) {
-                    const RT2 p2 = *reinterpret_cast<const RT2*>(XU + 6 * a + 4);
-                    const RT us[3] = {p1.y, p2.x, p2.y};
 #pragma unroll
                     for (int k = 0; k < 3; k++)
 #pragma unroll
@@ -348,16 +372,27 @@ __global__ __launch_bounds__(256) void free_hex20_kernel(const double* __restric
                 const RT gg = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
                 f[e] += (k1 * gi * gi + k3 * (gg - gi * gi)) * S[3];
             } else {
-                const RT2 p01 = *reinterpret_cast<const RT2*>(S + 4 * pi);
-                const RT p2 = S[4 * pi + 2];
-                f[e] += d0 * p01.x + d1 * p01.y + d2 * p2;
+                if constexpr (F32) {
+                    const float4 p4 = *reinterpret_cast<const float4*>(S + 4 * pi);
+                    f[e] += d0 * p4.x + d1 * p4.y + d2 * p4.z;
+                } else {
+                    const RT2 p01 = *reinterpret_cast<const RT2*>(S + 4 * pi);
+                    const RT p2 = S[4 * pi + 2];
+                    f[e] += d0 * p01.x + d1 * p01.y + d2 * p2;
+                }
             }
         };
         for (uint32_t q = 0; q < nq; q++) {
-            const RT2 t01 = *reinterpret_cast<const RT2*>(tab + q * kTabRow + 4 * pa);
-            const RT t2 = tab[q * kTabRow + 4 * pa + 2];
+            RT t0, t1, t2;
+            if constexpr (F32) {
+                const float4 t4 = *reinterpret_cast<const float4*>(tab + q * kTabRow + 4 * pa);
+                t0 = t4.x, t1 = t4.y, t2 = t4.z;
+            } else {
+                const RT2 t01 = *reinterpret_cast<const RT2*>(tab + q * kTabRow + 4 * pa);
+                t0 = t01.x, t1 = t01.y, t2 = tab[q * kTabRow + 4 * pa + 2];
+            }
 #pragma unroll
-            for (uint32_t e = 0; e < EW; e++) point(e, q, t01.x, t01.y, t2);
+            for (uint32_t e = 0; e < EW; e++) point(e, q, t0, t1, t2);
         }
         if (lane < 60) {
 #pragma unroll
