@@ -94,10 +94,21 @@ struct amx_server {
     std::map<uint64_t, Table> ebe;  // batched element launches, key: n | kt << 16 | f64 << 24
     Table gather[2], cg[2];         // [f64]
     std::vector<Job*> free_jobs;    // jobs on method-2 (FREE) operators: element kernels launched per job
+    // the per-job FREE element kernels run on kAux side streams (default
+    // priority) forked from and joined back into `stream` every iteration, so
+    // that the kernels of small jobs share the GPU
+    static constexpr int kAux = 4;
+    hipStream_t aux[kAux] = {};
+    hipEvent_t fork = nullptr, join[kAux] = {};
     ~amx_server()
     {
         jobs.clear();
         if (sc_host) (void)hipHostFree(sc_host);
+        for (int k = 0; k < kAux; k++) {
+            if (join[k]) (void)hipEventDestroy(join[k]);
+            if (aux[k]) (void)hipStreamDestroy(aux[k]);
+        }
+        if (fork) (void)hipEventDestroy(fork);
         if (stream) (void)hipStreamDestroy(stream);
     }
     amx::CgScalars* it_sc(const Job& j) { return sc_dev.as<amx::CgScalars>() + 2 * j.slot; }
@@ -224,10 +235,20 @@ void launch_iteration(amx_server& S)
         amx::launch_ebe_batched(n, kt, f64, t.dev_jobs.as<amx::EbeJob>(), t.dev_prefix.as<uint32_t>(), t.n, t.total,
                                 s);
     }
-    for (Job* jp : S.free_jobs) {  // element kernels from coordinates (FP64 or FP32), into the job's element rows
-        Op& op = jp->h->op;
-        amx::ebe_launch(op, jp->kt, jp->f64iter, op.bc_operator == AMX_BCOP_IDENTITY, 2, jp->p.p, nullptr, s,
-                        &S.it_sc(*jp)->done, jp->ve.p);
+    if (!S.free_jobs.empty()) {  // element kernels from coordinates (FP64 or FP32), into each job's element rows
+        const int na = std::min<int>(amx_server::kAux, (int)S.free_jobs.size());
+        AMX_HIP(hipEventRecord(S.fork, s));
+        for (int k = 0; k < na; k++) AMX_HIP(hipStreamWaitEvent(S.aux[k], S.fork, 0));
+        for (size_t i = 0; i < S.free_jobs.size(); i++) {
+            Job* jp = S.free_jobs[i];
+            Op& op = jp->h->op;
+            amx::ebe_launch(op, jp->kt, jp->f64iter, op.bc_operator == AMX_BCOP_IDENTITY, 2, jp->p.p, nullptr,
+                            S.aux[i % na], &S.it_sc(*jp)->done, jp->ve.p);
+        }
+        for (int k = 0; k < na; k++) {
+            AMX_HIP(hipEventRecord(S.join[k], S.aux[k]));
+            AMX_HIP(hipStreamWaitEvent(s, S.join[k], 0));
+        }
     }
     for (int f = 0; f < 2; f++) {
         Table& t = S.gather[f];
@@ -412,6 +433,11 @@ int amx_server_create(int device, int max_jobs, amx_server** out)
         S->device = device;
         S->max_jobs = max_jobs;
         AMX_HIP(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+        AMX_HIP(hipEventCreateWithFlags(&S->fork, hipEventDisableTiming));
+        for (int k = 0; k < amx_server::kAux; k++) {
+            AMX_HIP(hipStreamCreateWithFlags(&S->aux[k], hipStreamNonBlocking));
+            AMX_HIP(hipEventCreateWithFlags(&S->join[k], hipEventDisableTiming));
+        }
         S->sc_dev.alloc((size_t)max_jobs * 2 * sizeof(amx::CgScalars));
         AMX_HIP(hipMemset(S->sc_dev.p, 0, S->sc_dev.bytes));
         void* p = nullptr;
