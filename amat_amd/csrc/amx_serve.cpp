@@ -186,19 +186,21 @@ void rebuild_tables(amx_server& S)
             e.first.push_back(a);
             e.second.push_back(e.second.back() + grp.n_tiles);
         }
-        amx::GatherJob ga{};
-        ga.off = op.off.as<uint32_t>();
-        ga.pos = op.pos.as<uint32_t>();
-        ga.ve = j.ve.p;
-        ga.u = j.p.p;
-        ga.flag = op.flag.as<uint8_t>();
-        ga.coef = op.coef64.as<double>();
-        ga.v = j.q.p;
-        ga.n = j.n;
-        ga.bc = (uint32_t)op.bc_operator;
-        ga.skip = skip;
-        g[f].push_back(ga);
-        gp[f].push_back(gp[f].back() + (j.n + 255) / 256);
+        if (!op.is_free()) {  // method-2 jobs gather on their side stream (launch_iteration)
+            amx::GatherJob ga{};
+            ga.off = op.off.as<uint32_t>();
+            ga.pos = op.pos.as<uint32_t>();
+            ga.ve = j.ve.p;
+            ga.u = j.p.p;
+            ga.flag = op.flag.as<uint8_t>();
+            ga.coef = op.coef64.as<double>();
+            ga.v = j.q.p;
+            ga.n = j.n;
+            ga.bc = (uint32_t)op.bc_operator;
+            ga.skip = skip;
+            g[f].push_back(ga);
+            gp[f].push_back(gp[f].back() + (j.n + 255) / 256);
+        }
         amx::CgJob cj{};
         cj.x = j.xi;
         cj.r = j.r.p;
@@ -235,15 +237,19 @@ void launch_iteration(amx_server& S)
         amx::launch_ebe_batched(n, kt, f64, t.dev_jobs.as<amx::EbeJob>(), t.dev_prefix.as<uint32_t>(), t.n, t.total,
                                 s);
     }
-    if (!S.free_jobs.empty()) {  // element kernels from coordinates (FP64 or FP32), into each job's element rows
+    if (!S.free_jobs.empty()) {  // element kernels from coordinates (FP64 or FP32) and the job's gather
         const int na = std::min<int>(amx_server::kAux, (int)S.free_jobs.size());
         AMX_HIP(hipEventRecord(S.fork, s));
         for (int k = 0; k < na; k++) AMX_HIP(hipStreamWaitEvent(S.aux[k], S.fork, 0));
         for (size_t i = 0; i < S.free_jobs.size(); i++) {
             Job* jp = S.free_jobs[i];
             Op& op = jp->h->op;
+            const uint32_t* skip = &S.it_sc(*jp)->done;
             amx::ebe_launch(op, jp->kt, jp->f64iter, op.bc_operator == AMX_BCOP_IDENTITY, 2, jp->p.p, nullptr,
-                            S.aux[i % na], &S.it_sc(*jp)->done, jp->ve.p);
+                            S.aux[i % na], skip, jp->ve.p);
+            // the job's gather follows on the same stream (the arithmetic of the
+            // batched gather), so it overlaps the other jobs' element kernels
+            amx::gather_launch(op, jp->f64iter, op.bc_operator, jp->p.p, jp->q.p, S.aux[i % na], skip, jp->ve.p);
         }
         for (int k = 0; k < na; k++) {
             AMX_HIP(hipEventRecord(S.join[k], S.aux[k]));
