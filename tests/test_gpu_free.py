@@ -341,3 +341,43 @@ def test_free_rejections(torch_gpu):
         with pytest.raises(AmatError) as e:
             o3.finalize()
         assert e.value.name == "INVALID_ARGUMENT" and "element 1" in str(e.value)
+
+
+def test_free_jobs_on_side_streams_bit_identical(torch_gpu):
+    """More method-2 jobs than the server's side streams (FP64 and FP32
+    reliable-update jobs of three kinds, different sizes, joining late): every
+    served solution equals its single solve bit for bit."""
+    torch = torch_gpu
+    from amat_amd import serve
+    from amat_amd.amat import IR_RELIABLE
+    specs = [("hex8", 5, PREC.FP64), ("hex8", 7, int(PREC.FP32) | IR_RELIABLE), ("hex20", 2, PREC.FP64),
+             ("hex20", 3, int(PREC.FP32) | IR_RELIABLE), ("tet10", 2, PREC.FP64), ("tet10", 3, int(PREC.FP32) | IR_RELIABLE),
+             ("hex8", 4, int(PREC.FP32) | IR_RELIABLE)]
+    cases = []
+    for i, (kind, n, method) in enumerate(specs):
+        p = serve.build_problem(kind, n)
+        op = serve.build_operator(p, free=True)
+        b = serve.rhs_for(p, 60 + i)
+        db = torch.from_numpy(b).cuda()
+        x = torch.zeros_like(db)
+        if method == PREC.FP64:
+            st = op.cg(db, x, rtol=1e-10, max_it=20000)
+        else:
+            st = op.ir_cg(method, db, x, rtol=1e-10, inner_rtol=0.1, max_outer=60, max_inner=20000)
+        assert st.converged
+        cases.append((p, op, b, method, x.cpu().numpy(), st.iterations))
+    srv = Server(max_jobs=16)
+    ids = {}
+    for k, (p, op, b, method, _, _) in enumerate(cases):
+        ids[srv.submit(op, method, b, inner_rtol=0.1)] = k
+        if k == 3:
+            srv.step(8)   # the later jobs join while the first ones iterate
+    out = {}
+    while len(out) < len(cases):
+        srv.step(8)
+        for j in srv.poll():
+            out[j] = srv.result(j, cases[ids[j]][0].n_dofs)
+    for j, k in ids.items():
+        x, st = out[j]
+        assert st.converged and st.iterations == cases[k][5]
+        np.testing.assert_array_equal(x, cases[k][4])
