@@ -80,11 +80,11 @@ template <typename RT> __device__ __forceinline__ RT jac_inv_r(const RT J[3][3],
 // ---- Laplace elements: one lane per element slot ------------------------
 // xe: [NN*3][stride] (coordinate-major, slot fastest), qp: [nq][NN*3+1]
 // (dN_ad at 3a+d, weight last; uniform across lanes -> scalar loads).
-// RT: compute and vector type (FP64; FP32 for the inner solves of the
-// mixed-precision solvers).  Coordinates, factors and the table stay FP64 in
-// memory and are rounded on load.
+// RT: compute, coordinate and vector type (FP64; FP32 for the inner solves of
+// the mixed-precision solvers, reading an FP32 copy of the coordinates).
+// Factors and the table stay FP64 in memory and are rounded on load.
 template <int NN, typename RT, bool MASKED, bool ATOMIC, bool DIAG>
-__global__ __launch_bounds__(256) void free_laplace_kernel(const double* __restrict__ xe, const double* __restrict__ scale,
+__global__ __launch_bounds__(256) void free_laplace_kernel(const RT* __restrict__ xe, const double* __restrict__ scale,
                                                           const uint32_t* __restrict__ map, const RT* __restrict__ u,
                                                           RT* __restrict__ out, uint64_t ve_base, uint32_t rs,
                                                           uint32_t ss, uint32_t slot_begin, uint32_t n_slots,
@@ -184,7 +184,7 @@ template <int QP, typename RT> constexpr int h20_wave_values()
 }
 
 template <int QP, typename RT, bool MASKED, bool ATOMIC, bool DIAG>
-__global__ __launch_bounds__(256) void free_hex20_kernel(const double* __restrict__ xe, const double* __restrict__ scale,
+__global__ __launch_bounds__(256) void free_hex20_kernel(const RT* __restrict__ xe, const double* __restrict__ scale,
                                                         const uint32_t* __restrict__ map, const RT* __restrict__ u,
                                                         RT* __restrict__ out, uint64_t ve_base, uint32_t rs,
                                                         uint32_t ss, uint32_t slot_begin, uint32_t n_slots,
@@ -408,6 +408,17 @@ __global__ __launch_bounds__(256) void free_hex20_kernel(const double* __restric
     }
 }
 
+// coordinates in the compute type: FP64, or the FP32 copy built at finalize
+template <typename RT> const RT* coords(const FreeArgs& a)
+{
+    if constexpr (std::is_same<RT, float>::value) {
+        AMX_REQUIRE(a.xe32 != nullptr, AMX_INVALID_STATE, "no FP32 coordinates");
+        return a.xe32;
+    } else {
+        return a.xe;
+    }
+}
+
 template <int QP, typename RT>
 void launch_hex20(bool masked, bool atomic, bool diag, const FreeArgs& a, hipStream_t s)
 {
@@ -429,10 +440,11 @@ void launch_hex20(bool masked, bool atomic, bool diag, const FreeArgs& a, hipStr
     }
     const uint32_t need = (a.n_slots + per_block - 1) / per_block;
     const dim3 grid(std::min<uint32_t>(need, (uint32_t)res)), block(256);
+    const RT* x = coords<RT>(a);
     const RT* u = static_cast<const RT*>(a.u);
     RT* out = static_cast<RT*>(a.out);
 #define AMX_H20(M, A, D)                                                                                           \
-    free_hex20_kernel<QP, RT, M, A, D><<<grid, block, 0, s>>>(a.xe, a.scale, a.map, u, out, a.ve_base, a.ve_rs,     \
+    free_hex20_kernel<QP, RT, M, A, D><<<grid, block, 0, s>>>(x, a.scale, a.map, u, out, a.ve_base, a.ve_rs,     \
                                                               a.ve_ss, a.slot_begin, a.n_slots, a.qp, a.nq, a.c1,   \
                                                               a.c2, a.c3, a.skip)
     if constexpr (std::is_same<RT, double>::value) {
@@ -450,10 +462,11 @@ template <int NN, typename RT>
 void launch_laplace(bool masked, bool atomic, bool diag, const FreeArgs& a, hipStream_t s)
 {
     const dim3 grid((a.n_slots + 255) / 256), block(256);
+    const RT* x = coords<RT>(a);
     const RT* u = static_cast<const RT*>(a.u);
     RT* out = static_cast<RT*>(a.out);
 #define AMX_LAP(M, A, D)                                                                                           \
-    free_laplace_kernel<NN, RT, M, A, D><<<grid, block, 0, s>>>(a.xe, a.scale, a.map, u, out, a.ve_base, a.ve_rs,  \
+    free_laplace_kernel<NN, RT, M, A, D><<<grid, block, 0, s>>>(x, a.scale, a.map, u, out, a.ve_base, a.ve_rs,  \
                                                                 a.ve_ss, a.slot_begin, a.n_slots, a.stride, a.qp,   \
                                                                 a.nq, a.skip)
     if constexpr (std::is_same<RT, double>::value) {

@@ -99,6 +99,7 @@ void launch_gather_batched(bool vt_double, const GatherJob* jobs, const uint32_t
 // ---- method 2 (FREE): element kernels from coordinates (amx_free.hip) ---
 struct FreeArgs {
     const double* xe;        // Laplace kinds: [3*nodes][stride]; hex20: [slot][60]
+    const float* xe32;       // the same coordinates rounded to FP32 (the FP32 form reads these)
     const double* scale;     // [slot] stiffness factor (0 on padding slots)
     const uint32_t* map;     // as EbeArgs::map
     const void* u;           // FP64, or FP32 with f32

@@ -63,6 +63,8 @@ void upload_free(Op& op, Group& g, hipStream_t s)
         }
     }
     upload(g.xe, hx, s, op.device_bytes);
+    const std::vector<float> hx32(hx.begin(), hx.end());
+    upload(g.xe32, hx32, s, op.device_bytes);
     upload(g.fscale, hs, s, op.device_bytes);
     AMX_HIP(hipStreamSynchronize(s));  // host vectors leave scope
 }
@@ -72,6 +74,7 @@ FreeArgs free_args(const Op& op, const Group& g, const void* u, void* out, uint3
 {
     FreeArgs a;
     a.xe = g.xe.as<double>();
+    a.xe32 = g.xe32.as<float>();
     a.scale = g.fscale.as<double>();
     a.map = g.map.as<uint32_t>();
     a.u = u;
@@ -103,6 +106,7 @@ void finalize_op(Op& op)
         for (auto& k : g.K) k.release();
         g.kdiag16.release();
         g.xe.release();
+        g.xe32.release();
         g.fscale.release();
     }
     const bool free_m = op.is_free();

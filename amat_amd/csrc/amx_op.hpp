@@ -32,7 +32,7 @@ struct Group {
     DevBuf K[3];                      // tiled storage per KType (F64 always)
     DevBuf map;                       // [slot][r] local ids | identity-BC flag
     DevBuf kdiag16;                   // BF16 storage: FP32 corrected diagonal [slot][r]
-    DevBuf xe, fscale;                // FREE: coordinates and factors per slot
+    DevBuf xe, xe32, fscale;          // FREE: coordinates (FP64 and an FP32 copy) and factors per slot
 };
 
 struct Op {

@@ -216,7 +216,7 @@ class OperatorCache:
 def estimate_op_bytes(kind: str, n: int, free: bool = False) -> int:
     """Device bytes of an operator of this kind and size with FP64 + FP32 +
     BF16 storage (element matrices, maps, deterministic-scatter tables), or
-    with free=True the method-2 form (coordinates and a factor per element
+    with free=True the method-2 form (coordinates in FP64 and FP32 and a factor per element
     instead of matrices), used before the operator exists."""
     if kind == "hex8":
         E, nd, N, nodes = n ** 3, 8, (n + 1) ** 3, 8
@@ -225,7 +225,7 @@ def estimate_op_bytes(kind: str, n: int, free: bool = False) -> int:
     else:
         E, nd = 6 * n ** 3, 10
         N, nodes = (n + 1) ** 3 + 7 * n ** 3 + 3 * n * (n + 1) ** 2, 10
-    per_elem = 24 * nodes + 8 if free else nd * nd * 14
+    per_elem = 36 * nodes + 8 if free else nd * nd * 14
     return E * per_elem + E * nd * (4 + 4 + 4 + 8 + 4) + N * 40
 
 

@@ -171,6 +171,27 @@ def precision_mode(args) -> dict:
         log(f"[precision] {'RU' if reliable else 'IR'}-CG {prec.name}: {'replacements' if reliable else 'outer'} "
             f"{si.outer_iterations} inner {si.iterations}, {ti:.3f} s, relres {si.rel_residual:.2e}, "
             f"err vs FP64 {err:.2e}, speedup {t64 / ti:.2f}x")
+    # method 2 (FREE): the same operator recomputed in-kernel from the coordinates,
+    # FP32 inner solves with FP64 reliable updates (nothing stored per precision)
+    del op
+    torch.cuda.empty_cache()
+    fr = p.build_free_operator(aMatFree, Maps, bcop=BCOP.IDENTITY)
+    ms = time_matvec(lambda a, b_, st_: fr.matmult_f32_dev(PREC.FP32, a, b_, st_), v32, u32, args.steps, args.warmup,
+                     torch)
+    x = torch.zeros_like(db)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    si = fr.ir_cg(int(PREC.FP32) | IR_RELIABLE, db, x, rtol=1e-10, inner_rtol=args.ru_delta, max_outer=200,
+                  max_inner=20000, stream=s)
+    torch.cuda.synchronize()
+    ti = time.perf_counter() - t
+    err = float(np.linalg.norm(x.cpu().numpy() - xr) / np.linalg.norm(xr))
+    res["ru_cg_fp32_free"] = {"matvec_ms": ms, "seconds": ti, "outer": si.outer_iterations, "inner": si.iterations,
+                              "rel_residual": si.rel_residual, "converged": bool(si.converged),
+                              "rel_error_vs_fp64_solution": err, "speedup_vs_fp64_cg": t64 / ti}
+    log(f"[precision] method-2 FP32 matvec {ms:.3f} ms; RU-CG FP32 on method 2: replacements {si.outer_iterations} "
+        f"inner {si.iterations}, {ti:.3f} s, relres {si.rel_residual:.2e}, err vs FP64 {err:.2e}, "
+        f"speedup {t64 / ti:.2f}x")
     return res
 
 
