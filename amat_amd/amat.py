@@ -354,12 +354,16 @@ class aMatFree:
         call("amx_op_matvec", self._h, _ptr(v), _ptr(uu), int(bool(isGhosted)))
         return v
 
-    def MatMult_mf(self, u: np.ndarray) -> np.ndarray:
-        """aMatFree::MatMult_mf (include/aMatFree.hpp:3893-3941) on owned-length arrays."""
+    def MatMult_mf(self, u: np.ndarray, out: np.ndarray | None = None) -> np.ndarray:
+        """aMatFree::MatMult_mf (include/aMatFree.hpp:3893-3941) on owned-length arrays
+        (`out`: a contiguous float64 array of u's length to write v into)."""
         uu = _arr(u, np.float64)
-        vv = np.zeros_like(uu)
-        call("amx_op_matmult_mf", self._h, _ptr(vv), _ptr(uu))
-        return vv
+        if out is None:
+            out = np.zeros_like(uu)
+        elif out.dtype != np.float64 or not out.flags.c_contiguous or out.shape != uu.shape:
+            raise ValueError("out must be a contiguous float64 array of u's shape")
+        call("amx_op_matmult_mf", self._h, _ptr(out), _ptr(uu))
+        return out
 
     def get_diagonal(self) -> np.ndarray:
         """MatGetDiagonal_mf (include/aMatFree.hpp:3944-3974), owned length."""

@@ -108,6 +108,31 @@ def time_matvec(fn, v, u, steps, warmup, torch):
     return e0.elapsed_time(e1) / steps
 
 
+def host_boundary(op, u_dev, reps: int, torch) -> dict:
+    """MatMult_mf on host arrays (include/aMatFree.hpp:3893-3941, the PETSc shell
+    hook's form): copy u in, the matvec, copy v out, synchronized; pageable numpy
+    arrays and pinned ones.  Reported beside the device rate, never as `value`."""
+    import numpy as np
+    n = u_dev.numel()
+    out = {}
+    for kind in ("pageable", "pinned"):
+        if kind == "pinned":
+            tu = torch.empty(n, dtype=torch.float64, pin_memory=True)
+            tv = torch.empty(n, dtype=torch.float64, pin_memory=True)
+            u, v = tu.numpy(), tv.numpy()
+        else:
+            u, v = np.empty(n), np.empty(n)
+        u[:] = u_dev.cpu().numpy()
+        for _ in range(2):
+            op.MatMult_mf(u, out=v)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            op.MatMult_mf(u, out=v)
+        dt = (time.perf_counter() - t0) / reps
+        out[kind] = {"ms_per_matvec": dt * 1e3, "GDoF_s": n / dt / 1e9}
+    return out
+
+
 def precision_mode(args) -> dict:
     """Config C3: jittered tet10 Poisson (n^3 hexes x 6 tets), b = M f, identity
     BCs; FP64 CG vs FP64 iterative refinement with FP32 / BF16 element
@@ -530,6 +555,9 @@ def main() -> int:
     from amat_amd.amat import measure_read_bandwidth
     read_bw = measure_read_bandwidth(torch.cuda.current_device(), 2 << 30, 10)
     log(f"[bench] measured HBM read stream: {read_bw / 1e9:.0f} GB/s")
+    host = host_boundary(op, u, args.steps, torch)
+    log(f"[bench] host-array MatMult_mf (PCIe copies included): pageable {host['pageable']['ms_per_matvec']:.3f} ms, "
+        f"pinned {host['pinned']['ms_per_matvec']:.3f} ms")
     E, n, N = p.n_elem, p.dofs_per_elem, p.n_dofs
     elem_bytes = E * (n * n * 8 + 4 * n) + N * 8       # K + map + u read
     alg_bytes = info.alg_bytes_fp64                     # + v write: the whole matvec
@@ -556,6 +584,7 @@ def main() -> int:
                      "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": "ebe_tile_kernel<60,double>",
                      "measured_read_GBps": read_bw / 1e9, "frac_of_measured_read": achieved / read_bw,
                      "alg_bytes_per_launch": elem_bytes, "ms_per_launch": ms_elem, "gather_ms": ms_gather},
+        "host_arrays_pcie_inclusive": host,
     }
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

@@ -49,6 +49,11 @@ def test_c1_poisson_hex8_32_bitexact(torch_gpu):
     op.matvec(vh, u, True)
     np.testing.assert_array_equal(vh, v)
     np.testing.assert_array_equal(op.MatMult_mf(u), vm)
+    out = np.full_like(u, np.nan)
+    assert op.MatMult_mf(u, out=out) is out
+    np.testing.assert_array_equal(out, vm)
+    with pytest.raises(ValueError):
+        op.MatMult_mf(u, out=np.zeros(u.size + 1))
 
 
 def test_hex20_elasticity_bitexact_and_penalty(torch_gpu):
