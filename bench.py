@@ -134,7 +134,7 @@ def host_boundary(op, u_dev, reps: int, torch) -> dict:
 
 
 def precision_mode(args) -> dict:
-    """Config C3: jittered tet10 Poisson (n^3 hexes x 6 tets), b = M f, identity
+    """Precision: jittered tet10 Poisson (n^3 hexes x 6 tets), b = M f, identity
     BCs; FP64 CG vs FP64 iterative refinement with FP32 / BF16 element
     matrices (restarted per outer step, and with reliable updates), all to
     1e-10 relative residual, error against the FP64 solution."""
@@ -150,7 +150,7 @@ def precision_mode(args) -> dict:
     b = op.apply_bc(p.rhs)
     db = torch.from_numpy(b).cuda()
     s = torch.cuda.current_stream()
-    res = {"metric": "C3 precision: IR-CG vs FP64 CG to 1e-10", "elements": p.n_elem, "dofs": p.n_dofs}
+    res = {"metric": "precision: IR-CG vs FP64 CG to 1e-10, jittered tet10 Poisson", "elements": p.n_elem, "dofs": p.n_dofs}
     # matvec rates per storage
     E, n, N = p.n_elem, 10, p.n_dofs
     u64 = torch.from_numpy(p.random_u()).cuda()
@@ -319,7 +319,7 @@ MFMA_PEAK = {"fp32": 157.3e12, "bf16": 2.5e15}  # MI355X dense MFMA, flop/s
 
 
 def batch_mode(args) -> dict:
-    """Config C4: multi-RHS products of hex20 elasticity 48^3 and hex8 Poisson
+    """Batching: multi-RHS products of hex20 elasticity 48^3 and hex8 Poisson
     80x40x20 together in one persistent MFMA launch, nrhs in {8, 32, 128},
     FP32 and BF16 storage, FP32 vectors (n_total x nrhs, DoF-major).  Per case:
     autotuned tile shape, mean ms per launch (eager and HIP-graph replay),
@@ -334,7 +334,7 @@ def batch_mode(args) -> dict:
     log(f"[batch] operators: " + ", ".join(f"{p.name} E={p.n_elem} N={p.n_dofs}" for p in probs) +
         f" ({time.time() - t0:.1f}s)")
     s = torch.cuda.Stream()
-    res = {"metric": "C4 multi-RHS EbE products (hex20 48^3 + hex8 80x40x20, one persistent launch)",
+    res = {"metric": "multi-RHS EbE products (hex20 48^3 + hex8 80x40x20, one persistent launch)",
            "operators": [{"name": p.name, "elements": p.n_elem, "dofs": p.n_dofs, "dofs_per_element": p.dofs_per_elem}
                          for p in probs], "cases": []}
     scatters = {"atomic": SCATTER.ATOMIC, "deterministic": SCATTER.DETERMINISTIC}
@@ -395,7 +395,7 @@ def batch_mode(args) -> dict:
 
 
 def serve_mode(args) -> dict:
-    """Config C5: a stream of CG solve requests (60% hex8 Poisson n~U{16..96},
+    """Serving: a stream of CG solve requests (60% hex8 Poisson n~U{16..96},
     30% hex20 elasticity n~U{8..40}, 10% tet10 Poisson n~U{12..48}, own RHS,
     1e-10 relative residual) arriving as Poisson traffic, served by the
     continuous-batching server under an HBM budget; FP64 CG and FP64-refined
@@ -430,7 +430,7 @@ def serve_mode(args) -> dict:
             f"built in {cache.build_seconds:.1f}s")
         caches[free] = cache
     cache = caches.get(False, caches.get(True))
-    res = {"metric": "C5 served CG solves/s and latency at 1e-10 relative residual", "requests": args.serve_requests,
+    res = {"metric": "served CG solves/s and latency at 1e-10 relative residual", "requests": args.serve_requests,
            "rates": rates, "budget_GiB": args.serve_budget_gb, "slots": args.serve_slots, "step_iters": args.serve_iters,
            "inner_rtol": args.inner_rtol, "ru_delta": args.ru_delta,
            "operators": len(keys), "operator_GiB": cache.bytes / 2**30, "operator_build_s": cache.build_seconds,
@@ -499,8 +499,8 @@ def main() -> int:
                          "FETCH_SIZE / WRITE_SIZE counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU's share; --gpus {args.gpus} is not run here (multi-GPU runs are out of "
-              "scope for this build)", file=sys.stderr)
+        print(f"bench.py measures one GPU (one rank's share of a partitioned mesh); --gpus {args.gpus} is not "
+              "supported: multi-GPU runs are out of scope", file=sys.stderr)
         return 2
 
     import torch

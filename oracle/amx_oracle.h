@@ -1,7 +1,7 @@
 /*
  * amx_oracle.h -- CPU restatement of aMat's assembly-free matvec path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * TEST INFRASTRUCTURE ONLY.  This library is the test oracle: only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
  * product (amat_amd/) never links or calls it.
  *

@@ -59,7 +59,7 @@ def fem_fixtures() -> None:
         ke = np.zeros(64)
         L.ref_ke_hex8_eig(p(ke), p(x), C.c_uint(2))
         out[f"hex8eig_{name}_ke"] = ke.reshape(8, 8)
-    # hex20 geometries: the unit cube of config C2 (h = 1/64) and a distorted one
+    # hex20 geometries: the unit cube of the 64^3 elasticity workload (h = 1/64) and a distorted one
     m = meshmod.hex20(1, 1, 1)
     base = m.coords[m.elem_nodes[0]]
     h20 = {"cube64": base / 64.0, "cube1": base,

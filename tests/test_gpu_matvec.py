@@ -34,8 +34,8 @@ def build(p, bcop=BCOP.IDENTITY, precision=PREC.FP64, scatter=SCATTER.DETERMINIS
                             penalty=penalty)
 
 
-def test_c1_poisson_hex8_32_bitexact(torch_gpu):
-    """Config C1 at full size: 32^3 hex8 Poisson, identity BCs on the boundary."""
+def test_poisson_hex8_32_bitexact(torch_gpu):
+    """Full size: 32^3 hex8 Poisson, identity BCs on the boundary."""
     p = problems.poisson_hex8(32)
     op = build(p)
     u = p.random_u()
@@ -265,8 +265,8 @@ def test_two_rank_partition_with_halo_on_one_gpu(torch_gpu):
         assert rel_l2(got, full[r.owned_begin:r.owned_end + 1]) <= 1e-14
 
 
-def test_c2_full_size_bitexact(torch_gpu):
-    """Config C2 at full size: 64^3 hex20 elasticity (262,144 elements,
+def test_elasticity_hex20_64_bitexact(torch_gpu):
+    """Full size: 64^3 hex20 elasticity (262,144 elements,
     3,257,475 DoFs), K_e = E_e K_ref; the plain product is bit-exact against
     the oracle, the penalty operator agrees through the diagonal identity,
     and symmetry holds."""
