@@ -116,6 +116,7 @@ SIGNATURES = {
     "amx_op_get_diagonal": (I32, [P, P]),
     "amx_op_get_diagonal_dev": (I32, [P, P, P]),
     "amx_op_apply_bc_rhs": (I32, [P, P]),
+    "amx_op_get_timer": (I32, [P, P, P, P, P]),
     "amx_op_time_kernels_dev": (I32, [P, I32, P, P, P, P, P, P]),
     "amx_op_stream": (P, [P]),
     "amx_op_synchronize": (I32, [P]),

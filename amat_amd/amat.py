@@ -408,6 +408,15 @@ class aMatFree:
     def get_diagonal_dev(self, diag, stream=None):
         call("amx_op_get_diagonal_dev", self._h, _ptr(diag), _stream_ptr(stream))
 
+    def get_timer(self) -> tuple:
+        """aMatFree::get_timer (include/aMatFree.hpp:239-245): seconds
+        (scatter, gather, mv, total) accumulated since finalize by the
+        host-array forms (matvec, MatMult_mf): u copied in, v copied out, the
+        device application, the whole calls."""
+        s, g, m, t = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        call("amx_op_get_timer", self._h, C.byref(s), C.byref(g), C.byref(m), C.byref(t))
+        return s.value, g.value, m.value, t.value
+
     def time_kernels_dev(self, n_iter, v, u, stream=None):
         """Mean ms of the element kernels, the gather and the whole application
         over n_iter runs (hipEvents on the launch stream)."""

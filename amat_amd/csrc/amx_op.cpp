@@ -8,6 +8,7 @@
 // gather, and the BC tables.  Each matvec is then one EbE launch per group and
 // one gather launch; u and v never leave HBM.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 
 #include "amx_op.hpp"
@@ -339,6 +340,7 @@ void finalize_op(Op& op)
         AMX_HIP(hipStreamSynchronize(s));  // host vectors leave scope
     }
     AMX_HIP(hipStreamSynchronize(s));
+    op.t_scatter = op.t_mv = op.t_gather = op.t_total = 0.0;
     op.finalized = true;
 }
 
@@ -347,7 +349,33 @@ void ensure_scratch(Op& op)
     const size_t b = std::max<uint32_t>(op.n_total, 1) * sizeof(double);
     if (op.scratch_u.bytes < b) op.scratch_u.alloc(b);
     if (op.scratch_v.bytes < b) op.scratch_v.alloc(b);
+    for (hipEvent_t& e : op.tev)
+        if (!e) AMX_HIP(hipEventCreate(&e));
 }
+
+// Phases of a host-array matvec, accumulated as aMatFree::get_timer reports
+// them (include/aMatFree.hpp:239-245; the GPU path's timers in
+// matvec_ghosted_gpuPure, :3644-3700): u copied in ~ scatter_time (pack +
+// host-to-device), the device application ~ mv_time (the batched GEMV), v
+// copied out ~ gather_time (device-to-host + host accumulation), the whole
+// call on the host clock ~ mvTotal_time.
+struct HostCallTimer {
+    Op& op;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    explicit HostCallTimer(Op& o) : op(o) {}
+    void mark(int i, hipStream_t s) { AMX_HIP(hipEventRecord(op.tev[i], s)); }
+    void finish()  // after the stream has synchronized
+    {
+        float a = 0, b = 0, c = 0;
+        AMX_HIP(hipEventElapsedTime(&a, op.tev[0], op.tev[1]));
+        AMX_HIP(hipEventElapsedTime(&b, op.tev[1], op.tev[2]));
+        AMX_HIP(hipEventElapsedTime(&c, op.tev[2], op.tev[3]));
+        op.t_scatter += 1e-3 * a;
+        op.t_mv += 1e-3 * b;
+        op.t_gather += 1e-3 * c;
+        op.t_total += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
 
 // write one element block into the staging copy (copy_element_matrix :882-953)
 void stage_block(Op& op, uint32_t eid, const double* m, int layout, uint32_t bi, uint32_t bj, uint32_t bd)
@@ -845,16 +873,22 @@ int amx_op_matvec(amx_op* h, double* v, const double* u, int is_ghosted)
         hipStream_t s = op.stream;
         double* du = op.scratch_u.as<double>();
         double* dv = op.scratch_v.as<double>();
+        amx::HostCallTimer tm(op);
+        tm.mark(0, s);
         if (is_ghosted) {
             AMX_HIP(hipMemcpyAsync(du, u, op.n_total * sizeof(double), hipMemcpyHostToDevice, s));
         } else {  // local_to_ghost (:824-838): ghost slots are 0
             amx::launch_fill(du, op.n_total, true, 0.0, s);
             AMX_HIP(hipMemcpyAsync(du + op.n_pre, u, op.n_owned * sizeof(double), hipMemcpyHostToDevice, s));
         }
+        tm.mark(1, s);
         amx::apply(op, KType::F64, true, AMX_BCOP_NONE, du, dv, s);
+        tm.mark(2, s);
         if (is_ghosted) AMX_HIP(hipMemcpyAsync(v, dv, op.n_total * sizeof(double), hipMemcpyDeviceToHost, s));
         else AMX_HIP(hipMemcpyAsync(v, dv + op.n_pre, op.n_owned * sizeof(double), hipMemcpyDeviceToHost, s));
+        tm.mark(3, s);
         AMX_HIP(hipStreamSynchronize(s));
+        tm.finish();
     });
 }
 
@@ -868,10 +902,29 @@ int amx_op_matmult_mf(amx_op* h, double* vv, const double* uu)
         DeviceGuard dg(op.device);
         amx::ensure_scratch(op);
         hipStream_t s = op.stream;
+        amx::HostCallTimer tm(op);
+        tm.mark(0, s);
         AMX_HIP(hipMemcpyAsync(op.scratch_u.p, uu, op.n_owned * sizeof(double), hipMemcpyHostToDevice, s));
+        tm.mark(1, s);
         amx::apply(op, KType::F64, true, op.bc_operator, op.scratch_u.p, op.scratch_v.p, s);
+        tm.mark(2, s);
         AMX_HIP(hipMemcpyAsync(vv, op.scratch_v.p, op.n_owned * sizeof(double), hipMemcpyDeviceToHost, s));
+        tm.mark(3, s);
         AMX_HIP(hipStreamSynchronize(s));
+        tm.finish();
+    });
+}
+
+int amx_op_get_timer(amx_op* h, double* scatter_s, double* gather_s, double* mv_s, double* total_s)
+{
+    return guarded(__func__, [&] {
+        AMX_REQUIRE(h && scatter_s && gather_s && mv_s && total_s, AMX_INVALID_ARGUMENT, "NULL argument");
+        const Op& op = h->op;
+        amx::require_finalized(op);
+        *scatter_s = op.t_scatter;
+        *gather_s = op.t_gather;
+        *mv_s = op.t_mv;
+        *total_s = op.t_total;
     });
 }
 
@@ -949,8 +1002,20 @@ int amx_op_time_kernels_dev(amx_op* h, int n_iter, double* v, const double* u, d
         AMX_REQUIRE(op.size == 1, AMX_NOT_IMPLEMENTED, "timers run the one-rank operator");
         DeviceGuard dg(op.device);
         hipStream_t s = op.pick(stream);
-        std::vector<hipEvent_t> ev(3 * (size_t)n_iter);
-        for (auto& e : ev) AMX_HIP(hipEventCreate(&e));
+        struct Events {  // destroyed on every exit, errors included
+            std::vector<hipEvent_t> v;
+            ~Events()
+            {
+                for (hipEvent_t e : v) (void)hipEventDestroy(e);
+            }
+        } evs;
+        evs.v.reserve(3 * (size_t)n_iter);
+        for (int i = 0; i < 3 * n_iter; i++) {
+            hipEvent_t e = nullptr;
+            AMX_HIP(hipEventCreate(&e));
+            evs.v.push_back(e);
+        }
+        std::vector<hipEvent_t>& ev = evs.v;
         const int bc = op.bc_operator;
         const bool masked = bc == AMX_BCOP_IDENTITY;
         for (int i = 0; i < n_iter; i++) {
@@ -974,7 +1039,6 @@ int amx_op_time_kernels_dev(amx_op* h, int n_iter, double* v, const double* u, d
             tg += b;
             tt += a + b;
         }
-        for (auto& e : ev) (void)hipEventDestroy(e);
         if (ms_element) *ms_element = te / n_iter;
         if (ms_gather) *ms_gather = tg / n_iter;
         if (ms_total) *ms_total = tt / n_iter;

@@ -87,6 +87,10 @@ struct Op {
     DevBuf halo_dst, halo_off, halo_src;
     uint32_t n_halo_unique = 0;
     DevBuf scratch_u, scratch_v;         // host-array API staging
+    // host-array matvec timers (aMatFree::get_timer): events of the phases
+    // (created with the staging buffers) and seconds accumulated since finalize
+    hipEvent_t tev[4] = {};
+    double t_scatter = 0, t_mv = 0, t_gather = 0, t_total = 0;
     // solver workspace (allocated on first solve)
     DevBuf ws_r, ws_z, ws_p, ws_q, ws_dinv, ws_partials, ws_sc, ws_x32, ws_b32, ws_r32, ws_z32, ws_p32, ws_q32;
     DevBuf ws_ib, ws_ix, ws_ir, ws_iz, ws_ip, ws_iq;  // refinement with FP64 inner vectors
@@ -99,6 +103,8 @@ struct Op {
     ~Op()
     {
         if (host_sc) (void)hipHostFree(host_sc);
+        for (hipEvent_t e : tev)
+            if (e) (void)hipEventDestroy(e);
     }
     double penalty() const { return penalty_set ? penalty_value : kPenaltyFactor * trace; }
     // the caller's stream; NULL is the HIP default stream (not the handle's)

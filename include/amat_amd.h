@@ -317,6 +317,16 @@ int amx_op_get_diagonal_dev(amx_op* op, double* diag, void* stream);
  * prescribed value) and, for BC_IMATRIX, free rows reduced by K_fc U_c. */
 int amx_op_apply_bc_rhs(amx_op* op, double* rhs);
 
+/* aMatFree::get_timer (include/aMatFree.hpp:239-245): seconds accumulated
+ * since finalize by the host-array forms (amx_op_matvec, amx_op_matmult_mf),
+ * in the reference's argument order.  scatter = u copied to the device (the
+ * reference: pack + host-to-device), mv = the device application (element
+ * kernels, gather, BC rows; the reference: the batched GEMV), gather = v
+ * copied back (the reference: device-to-host + host accumulation), total =
+ * the whole calls on the host clock.  Device-array forms are not counted
+ * (amx_op_time_kernels_dev times those). */
+int amx_op_get_timer(amx_op* op, double* scatter_s, double* gather_s, double* mv_s, double* total_s);
+
 /* Kernel timers, the GPU counterpart of aMatFree::get_timer
  * (include/aMatFree.hpp:239-245, mv_time / gather_time): runs n_iter
  * applications of the configured BC form on `stream` with hipEvents around

@@ -200,6 +200,19 @@ template <typename DT, typename GI, typename LI> class aMatFree : public aMat<aM
         detail::check(amx_op_set_num_streams(m_h, nStreams), "set_num_streams");
         return Error::SUCCESS;
     }
+    /* aMatFree::get_timer (include/aMatFree.hpp:239-245): seconds accumulated
+     * since finalize by the host-array matvec / MatMult_mf calls (u copied in,
+     * v copied out, the device application, the whole calls) */
+    Error get_timer(long double* scatterTime, long double* gatherTime, long double* mvTime, long double* mvTotalTime)
+    {
+        double s = 0, g = 0, m = 0, t = 0;
+        detail::check(amx_op_get_timer(m_h, &s, &g, &m, &t), "get_timer");
+        *scatterTime = s;
+        *gatherTime = g;
+        *mvTime = m;
+        *mvTotalTime = t;
+        return Error::SUCCESS;
+    }
     /* aMatFree::set_element_matrix (include/aMatFree.hpp:854-880); MatrixType
      * needs rows(), cols() and operator()(r, c) (an Eigen matrix works) */
     template <typename MatrixType>

@@ -24,6 +24,7 @@ def test_cpp_dropin_example(tmp_path):
     assert r.stdout.count("maxdiff_matvec=0.000e+00 maxdiff_mf=0.000e+00") == 3
     assert "dropin gmsh_tet10_162 maxdiff_matvec=0.000e+00 maxdiff_mf=0.000e+00 constraints=218" in r.stdout
     assert "error-path code=11" in r.stdout
+    assert "timers=ok" in r.stdout and "timers=bad" not in r.stdout
     # method 2: the callback form is stored (bit-exact), the in-kernel form within 1e-13 of max |v|
     assert "dropin free_callback_7x6x5 maxdiff_matvec=0.000e+00" in r.stdout
     rel = float(r.stdout.split("free_inkernel_7x6x5 relmax=")[1].split()[0])

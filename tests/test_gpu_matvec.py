@@ -54,6 +54,15 @@ def test_poisson_hex8_32_bitexact(torch_gpu):
     np.testing.assert_array_equal(out, vm)
     with pytest.raises(ValueError):
         op.MatMult_mf(u, out=np.zeros(u.size + 1))
+    # aMatFree::get_timer: the host-array calls above (matvec, 2 x MatMult_mf) accumulated
+    sc, ga, mv, tot = op.get_timer()
+    assert sc > 0 and ga > 0 and mv > 0 and tot >= sc + mv + ga > 0
+    op.MatMult_mf(u, out=out)
+    sc2, ga2, mv2, tot2 = op.get_timer()
+    assert sc2 > sc and ga2 > ga and mv2 > mv and tot2 > tot
+    # device forms are not counted
+    run(torch_gpu, op.matmult_dev, u)
+    assert op.get_timer() == (sc2, ga2, mv2, tot2)
 
 
 def test_hex20_elasticity_bitexact_and_penalty(torch_gpu):
