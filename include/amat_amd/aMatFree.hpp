@@ -25,12 +25,16 @@
  * set_element_matrix_function callback, which is host code: it is evaluated
  * once per element at finalize and its matrices are stored (the operator then
  * runs as method 3, with the callback's values).  The PETSc shell hook
- * (get_matrix) is compiled only with BUILD_WITH_PETSC.
+ * (get_matrix) and apply_bc(Vec) are compiled only with BUILD_WITH_PETSC;
+ * dump_mat writes "row col value" lines instead of PETSc's MatView.
  */
 #ifndef AMAT_AMD_AMATFREE_HPP
 #define AMAT_AMD_AMATFREE_HPP
 
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -53,6 +57,8 @@ enum class Error {
 };
 
 enum class MATFREE_TYPE { UNDEFINED = 0, QUASI_FREE = 1, FREE = 2, GPU_PURE = 3, GPU_OVER_CPU = 4, GPU_OVER_GPU = 5 };
+
+enum class MATRIX_TYPE { MATRIX_BASED, MATRIX_FREE };
 
 struct amx_error : std::runtime_error {
     int code;
@@ -145,6 +151,8 @@ template <typename Derived, typename DT, typename GI, typename LI> class aMat {
     typedef DT DTType;
     typedef GI GIType;
     typedef LI LIType;
+    // virtual: drivers delete the operator through this base (example07a.cpp:1128)
+    virtual ~aMat() = default;
     Error set_matfree_type(MATFREE_TYPE t) { return static_cast<Derived*>(this)->set_matfree_type_impl(t); }
     template <typename MatrixType>
     Error set_element_matrix(LI eid, const MatrixType& e_mat, LI block_i, LI block_j, LI blocks_dim)
@@ -156,6 +164,27 @@ template <typename Derived, typename DT, typename GI, typename LI> class aMat {
     {
         return static_cast<Derived*>(this)->matvec_impl(v, u, isGhosted);
     }
+    // the rest of the reference facade (include/aMat.hpp:114-196), so that
+    // drivers holding a par::aMat<par::aMatFree<...>, ...>* (as
+    // examples/src/example07a.cpp:730-745 does) compile unchanged
+    void set_element_matrix_function(void (*eMat)(LI, DT*, DT*))
+    {
+        static_cast<Derived*>(this)->set_element_matrix_function(eMat);
+    }
+    MATRIX_TYPE get_matrix_type() { return static_cast<Derived*>(this)->get_matrix_type(); }
+    Error apply_bc(DT* rhs) { return static_cast<Derived*>(this)->apply_bc(rhs); }
+    Error finalize_begin() { return static_cast<Derived*>(this)->finalize_begin(); }
+    Error finalize_end() { return static_cast<Derived*>(this)->finalize_end(); }
+    Error dump_mat(const char* filename = nullptr) { return static_cast<Derived*>(this)->dump_mat(filename); }
+    Error set_num_streams(LI nStreams) { return static_cast<Derived*>(this)->set_num_streams(nStreams); }
+    Error get_timer(long double* scatterTime, long double* gatherTime, long double* mvTime, long double* mvTotalTime)
+    {
+        return static_cast<Derived*>(this)->get_timer(scatterTime, gatherTime, mvTime, mvTotalTime);
+    }
+#ifdef BUILD_WITH_PETSC
+    Mat& get_matrix() { return static_cast<Derived*>(this)->get_matrix(); }
+    Error apply_bc(Vec rhs) { return static_cast<Derived*>(this)->apply_bc(rhs); }
+#endif
 };
 
 template <typename DT, typename GI, typename LI> class aMatFree : public aMat<aMatFree<DT, GI, LI>, DT, GI, LI> {
@@ -259,6 +288,52 @@ template <typename DT, typename GI, typename LI> class aMatFree : public aMat<aM
         detail::check(amx_op_finalize(m_h), "finalize");
         return Error::SUCCESS;
     }
+    /* aMatFree::finalize_begin / finalize_end (include/aMatFree.hpp:656-785):
+     * the whole set-up (device upload, trace) happens in finalize_begin */
+    Error finalize_begin() { return finalize_impl(); }
+    Error finalize_end()
+    {
+        if (!info().finalized) throw amx_error(AMX_INVALID_STATE, "finalize_end: finalize_begin was not called");
+        return Error::SUCCESS;
+    }
+    /* aMatFree::allocate_matrix / update_matrix (include/aMatFree.hpp:574-652)
+     * size the host work vectors; here finalize sizes every buffer */
+    Error allocate_matrix() { return Error::SUCCESS; }
+    Error update_matrix() { return Error::SUCCESS; }
+    MATRIX_TYPE get_matrix_type() const { return MATRIX_TYPE::MATRIX_FREE; }
+    /* aMatFree::dump_mat (include/aMatFree.hpp:4737-4830): the matrix of
+     * matvec (no boundary conditions), probed column by column with unit
+     * vectors as the reference does, on one rank.  Written as lines
+     * "row col value" (global ids, entries with |value| > 1e-16, by row then
+     * column, values to 17 digits) to `filename`, or to stdout when it is
+     * NULL, where the reference prints PETSc's MatView of the same entries. */
+    Error dump_mat(const char* filename = nullptr)
+    {
+        const amx_maps_info mi = m_maps.info();
+        if (mi.size != 1) throw amx_error(AMX_NOT_IMPLEMENTED, "dump_mat: one rank");
+        const LI n = mi.n_total;
+        struct Entry {
+            uint64_t r, c;
+            double v;
+            bool operator<(const Entry& o) const { return r != o.r ? r < o.r : c < o.c; }
+        };
+        std::vector<Entry> entries;
+        std::vector<DT> ui(n, DT(0)), vi(n, DT(0));
+        for (LI j = 0; j < n; j++) {
+            ui[j] = DT(1);
+            matvec_impl(vi.data(), ui.data(), true);
+            ui[j] = DT(0);
+            for (LI r = 0; r < n; r++)
+                if (std::fabs(vi[r]) > 1e-16) entries.push_back({mi.owned_begin + r, mi.owned_begin + j, vi[r]});
+        }
+        std::sort(entries.begin(), entries.end());
+        std::FILE* f = filename ? std::fopen(filename, "w") : stdout;
+        if (!f) throw amx_error(AMX_INVALID_ARGUMENT, std::string("dump_mat: cannot open ") + filename);
+        for (const Entry& e : entries)
+            std::fprintf(f, "%llu %llu %.17g\n", (unsigned long long)e.r, (unsigned long long)e.c, e.v);
+        if (filename) std::fclose(f);
+        return Error::SUCCESS;
+    }
     /* aMatFree::matvec (include/aMatFree.hpp:1681-1775), host arrays */
     Error matvec_impl(DT* v, const DT* u, bool isGhosted)
     {
@@ -304,6 +379,16 @@ template <typename DT, typename GI, typename LI> class aMatFree : public aMat<aM
     amx_op* handle() { return m_h; }
 
 #ifdef BUILD_WITH_PETSC
+    /* aMatFree::apply_bc(Vec) (include/aMatFree.hpp:957-971), one rank */
+    Error apply_bc(Vec rhs)
+    {
+        PetscScalar* r;
+        VecGetArray(rhs, &r);
+        const int rc = amx_op_apply_bc_rhs(m_h, r);
+        VecRestoreArray(rhs, &r);
+        detail::check(rc, "apply_bc");
+        return Error::SUCCESS;
+    }
     /* aMatFree::get_matrix (include/aMatFree.hpp:535-570): a MATSHELL whose
      * MATOP_MULT / MATOP_GET_DIAGONAL run on the GPU (one rank). */
     Mat& get_matrix()

@@ -6,7 +6,9 @@
 // Prints one line "dropin <case> maxdiff_matvec=... maxdiff_mf=..." per case
 // and exits non-zero on any mismatch.  With a Gmsh file as argument it also
 // runs the set-up of examples/src/example05a.cpp:135-246 (tet10 Laplace read
-// by GmshMesh) on one rank.
+// by GmshMesh) on one rank; with a second argument, the driver pattern of
+// examples/src/example07a.cpp (the operator held through the par::aMat base,
+// dump_mat into that file, get_timer).
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -22,7 +24,7 @@ struct Mat8 {  // anything with rows(), cols(), operator()(r, c) works
     double operator()(long r, long c) const { return a[r * 8 + c]; }
 };
 
-static int run_case(unsigned Nex, unsigned Ney, unsigned Nez)
+static int run_case(unsigned Nex, unsigned Ney, unsigned Nez, const char* dump_path = nullptr)
 {
     using Maps = par::Maps<double, unsigned long, unsigned int>;
     using AFM = par::aMatFree<double, unsigned long, unsigned int>;
@@ -98,7 +100,61 @@ static int run_case(unsigned Nex, unsigned Ney, unsigned Nez)
     }
     std::printf("dropin hex8_%ux%ux%u maxdiff_matvec=%.3e maxdiff_mf=%.3e timers=%s\n", Nex, Ney, Nez, d1, d2,
                 timers ? "ok" : "bad");
-    return (d1 == 0.0 && d2 == 0.0 && timers) ? 0 : 1;
+    int bad = (d1 == 0.0 && d2 == 0.0 && timers) ? 0 : 1;
+    if (!dump_path) return bad;
+
+    // the driver pattern of examples/src/example07a.cpp:729-745, 796-815, 880,
+    // 903-905, 981, 1128: the operator held through the par::aMat base
+    typedef par::aMat<AFM, double, unsigned long, unsigned int> aMatFreeBase;
+    aMatFreeBase* stMatFree = new AFM(maps, par::BC_METH::BC_IMATRIX);
+    stMatFree->set_matfree_type(par::MATFREE_TYPE::GPU_PURE);
+    stMatFree->set_num_streams(1);
+    for (unsigned e = 0; e < nelem; e++) {
+        for (int q = 0; q < 64; q++) ke.a[q] = Kall[(size_t)e * 64 + q];
+        stMatFree->set_element_matrix(e, ke, 0, 0, 1);
+    }
+    stMatFree->finalize();
+    std::vector<double> vb(nnode);
+    stMatFree->matvec(vb.data(), u.data(), true);
+    stMatFree->dump_mat(dump_path);
+    long double st = 0, gt = 0, mt = 0, tt = 0;
+    stMatFree->get_timer(&st, &gt, &mt, &tt);
+    const bool base_ok = vb == v && st > 0 && gt > 0 && mt > 0 && tt >= mt &&
+                         stMatFree->get_matrix_type() == par::MATRIX_TYPE::MATRIX_FREE;
+    delete stMatFree;
+    // the dump: symmetric bit for bit (symmetric K_e, the same element order in
+    // both entries) and its product with u equals matvec
+    std::vector<std::vector<std::pair<unsigned, double>>> rows(nnode);
+    std::FILE* f = std::fopen(dump_path, "r");
+    size_t n_entries = 0;
+    bool sym = f != nullptr;
+    if (f) {
+        unsigned long long r, c;
+        double val;
+        while (std::fscanf(f, "%llu %llu %lg", &r, &c, &val) == 3) {
+            if (r >= nnode || c >= nnode) { sym = false; break; }
+            rows[r].push_back({(unsigned)c, val});
+            n_entries++;
+        }
+        std::fclose(f);
+    }
+    double err = 0, vmax = 0;
+    for (unsigned r = 0; r < nnode && sym; r++) {
+        double y = 0;
+        for (auto& e : rows[r]) {
+            y += e.second * u[e.first];
+            bool found = false;
+            for (auto& t : rows[e.first]) found |= (t.first == r && t.second == e.second);
+            sym &= found;
+        }
+        err = std::fmax(err, std::fabs(y - v[r]));
+        vmax = std::fmax(vmax, std::fabs(v[r]));
+    }
+    const double rel = err / vmax;
+    std::printf("dropin base_pointer_%ux%ux%u entries=%zu symmetric=%s dump_apply_rel=%.3e base=%s\n", Nex, Ney, Nez,
+                n_entries, sym ? "ok" : "bad", rel, base_ok ? "ok" : "bad");
+    bad |= (sym && base_ok && n_entries > 0 && rel <= 1e-13) ? 0 : 1;
+    return bad;
 }
 
 // method 2 (FREE) the way examples/src/example07a.cpp:54-70 sets it up: an
@@ -266,7 +322,7 @@ int main(int argc, char** argv)
     int bad = 0;
     try {
         if (argc > 1) bad |= run_gmsh_case(argv[1]);
-        bad |= run_case(4, 5, 6);
+        bad |= run_case(4, 5, 6, argc > 2 ? argv[2] : nullptr);
         bad |= run_case(16, 16, 16);
         bad |= run_free_case(7, 6, 5);
         // error path: methods 1/2 belong to the reference's CPU loops
