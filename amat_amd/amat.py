@@ -71,6 +71,12 @@ IR_F64_VECTORS = 0x100
 IR_RELIABLE = 0x200
 
 
+class MATRIX_TYPE(enum.IntEnum):
+    """par::MATRIX_TYPE (include/enums.hpp:52-53)."""
+    MATRIX_BASED = 0
+    MATRIX_FREE = 1
+
+
 class ELEM(enum.IntEnum):
     """Element kinds of method 2 (FREE), computed in-kernel from coordinates."""
     HEX8_LAPLACE = 1
@@ -339,6 +345,50 @@ class aMatFree:
 
     def finalize(self):
         call("amx_op_finalize", self._h)
+
+    def finalize_begin(self):
+        """aMatFree::finalize_begin (include/aMatFree.hpp:656-745): the whole set-up."""
+        self.finalize()
+
+    def finalize_end(self):
+        """aMatFree::finalize_end (include/aMatFree.hpp:748-785): the trace is
+        computed in finalize_begin; checks that it ran."""
+        if not self.info().finalized:
+            raise AmatError(Error.INVALID_STATE, "finalize_end: finalize_begin was not called")
+
+    def get_matrix_type(self) -> MATRIX_TYPE:
+        return MATRIX_TYPE.MATRIX_FREE
+
+    def dump_mat(self, filename=None) -> np.ndarray:
+        """aMatFree::dump_mat (include/aMatFree.hpp:4737-4830) on one rank: the
+        matrix of matvec (no boundary conditions) probed column by column with
+        unit vectors.  Returns (row, col, value) triplets with |value| > 1e-16
+        sorted by row then column, global ids; with a filename also writes them
+        as "row col value" lines (the reference prints PETSc's MatView)."""
+        mi = self.maps.info()
+        if mi.size != 1:
+            raise AmatError(Error.NOT_IMPLEMENTED, "dump_mat: one rank")
+        n, g0 = mi.n_total, int(mi.owned_begin)
+        ui, vi = np.zeros(n), np.zeros(n)
+        rows, cols, vals = [], [], []
+        for j in range(n):
+            ui[j] = 1.0
+            self.matvec(vi, ui, True)
+            ui[j] = 0.0
+            nz = np.nonzero(np.abs(vi) > 1e-16)[0]
+            rows.append(nz)
+            cols.append(np.full(nz.size, j))
+            vals.append(vi[nz].copy())
+        r = (np.concatenate(rows) if rows else np.zeros(0, np.int64)) + g0
+        c = (np.concatenate(cols) if cols else np.zeros(0, np.int64)) + g0
+        v = np.concatenate(vals) if vals else np.zeros(0)
+        order = np.lexsort((c, r))
+        trip = np.rec.fromarrays([r[order], c[order], v[order]], names="row,col,value")
+        if filename is not None:
+            with open(filename, "w") as f:
+                for t in trip:
+                    f.write(f"{int(t.row)} {int(t.col)} {float(t.value)!r}\n")
+        return trip
 
     def info(self) -> _capi.OpInfo:
         inf = _capi.OpInfo()

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle as orc
-from amat_amd import BCOP, PREC, SCATTER, Maps, aMatFree, mesh, problems
+from amat_amd import BCOP, MATRIX_TYPE, PREC, SCATTER, Maps, aMatFree, mesh, problems
 from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -63,6 +63,33 @@ def test_poisson_hex8_32_bitexact(torch_gpu):
     # device forms are not counted
     run(torch_gpu, op.matmult_dev, u)
     assert op.get_timer() == (sc2, ga2, mv2, tot2)
+
+
+def test_dump_mat_matches_assembled_matrix(torch_gpu, tmp_path):
+    """aMatFree::dump_mat: probing matvec with unit vectors gives the matrix
+    assembled from the element matrices (scipy COO, duplicates summed), with
+    boundary rows untouched (matvec applies no BCs)."""
+    import scipy.sparse as sp
+    p = problems.poisson_tet10(2, with_rhs=False)
+    op = build(p)
+    op.finalize_end()
+    assert op.get_matrix_type() == MATRIX_TYPE.MATRIX_FREE
+    K = p.element_matrices()
+    trip = op.dump_mat(tmp_path / "m.out")
+    n = p.n_dofs
+    gm = p.gmap.astype(np.int64)
+    E, ne = gm.shape
+    R = np.broadcast_to(gm[:, :, None], (E, ne, ne)).ravel()
+    Cc = np.broadcast_to(gm[:, None, :], (E, ne, ne)).ravel()
+    A = sp.coo_matrix((K.reshape(-1), (R, Cc)), shape=(n, n)).tocsr()
+    D = sp.coo_matrix((trip.value, (trip.row, trip.col)), shape=(n, n)).tocsr()
+    assert abs(A - D).max() <= 1e-14 * abs(A).max()
+    assert (D != D.T).nnz == 0          # symmetric K_e, same element order: symmetric bit for bit
+    assert np.all(np.diff(trip.row) >= 0)
+    lines = (tmp_path / "m.out").read_text().splitlines()
+    assert len(lines) == len(trip)
+    r0, c0, v0 = lines[0].split()
+    assert (int(r0), int(c0), float(v0)) == (int(trip.row[0]), int(trip.col[0]), float(trip.value[0]))
 
 
 def test_hex20_elasticity_bitexact_and_penalty(torch_gpu):
