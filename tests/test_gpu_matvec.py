@@ -191,6 +191,25 @@ def test_mixed_element_sizes_and_generic_kernel(torch_gpu):
     np.testing.assert_array_equal(v, vo)
 
 
+def test_operator_without_elements(torch_gpu):
+    """A rank that owns DoFs but no elements: K = 0, so matvec gives 0 and
+    MatMult_mf only its identity rows v_c = u_c (include/aMatFree.hpp:3916-3933)."""
+    m = Maps()
+    m.set_map(0, np.zeros((0, 8), np.uint32), np.zeros(0, np.uint32), 4, np.arange(4), 0, 3, 4)
+    m.set_bdr_map([1], [0.0])
+    op = aMatFree(m)
+    op.set_matfree_type(3)
+    op.finalize()
+    assert op.info().n_elems == 0
+    u = np.array([1.0, 2.0, 3.0, 4.0])
+    np.testing.assert_array_equal(run(torch_gpu, op.matvec_dev, u), np.zeros(4))
+    np.testing.assert_array_equal(run(torch_gpu, op.matmult_dev, u), [0.0, 2.0, 0.0, 0.0])
+    np.testing.assert_array_equal(op.MatMult_mf(u), [0.0, 2.0, 0.0, 0.0])
+    # apply_bc_diagonal walks the elements' BC masks (include/aMatFree.hpp:4309-4337):
+    # a constrained DoF that no element touches keeps its assembled diagonal, 0
+    np.testing.assert_array_equal(op.get_diagonal(), np.zeros(4))
+
+
 def test_block_element_matrices(torch_gpu):
     rng = np.random.default_rng(4)
     E, bd, nb, N = 40, 2, 4, 90
