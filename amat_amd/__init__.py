@@ -5,9 +5,8 @@ host C++ and gfx950 HIP kernels).  This package holds its Python binding
 (Maps, aMatFree mirroring paralab/aMat's classes), mesh and workload builders,
 and the build script.
 """
-from .amat import (BC_METH, BCOP, ELEM, MATFREE_TYPE, MATRIX_TYPE, PREC, SCATTER, AmatError, Error, Maps, MultiRHS,
-                   aMatFree, device_count, version)
+from .amat import (BC_METH, BCOP, ELEM, GHOST_TO_OWNER, HALO, MATFREE_TYPE, MATRIX_TYPE, OWNER_TO_GHOST, PREC,
+                   SCATTER, AmatError, Error, Maps, MultiRHS, aMatFree, device_count, version)
 
-__all__ = ["AmatError", "BC_METH", "BCOP", "ELEM", "Error", "MATFREE_TYPE", "MATRIX_TYPE", "Maps", "MultiRHS", "PREC",
-           "SCATTER",
-           "aMatFree", "device_count", "version"]
+__all__ = ["AmatError", "BC_METH", "BCOP", "ELEM", "Error", "GHOST_TO_OWNER", "HALO", "MATFREE_TYPE", "MATRIX_TYPE",
+           "Maps", "MultiRHS", "OWNER_TO_GHOST", "PREC", "SCATTER", "aMatFree", "device_count", "version"]

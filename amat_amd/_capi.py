@@ -44,6 +44,10 @@ class GmshInfo(C.Structure):
 DISPLACEMENT_FN = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.POINTER(C.c_double))
 
 
+# fn(ctx, direction, send, recv, stream) of amx_op_set_halo_transport
+HALO_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p)
+
+
 class ServerInfo(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("active_jobs", "held_jobs", "free_slots", "pad")] + [
         (n, C.c_uint64) for n in ("workspace_bytes", "steps", "iterations", "completed")]
@@ -102,12 +106,14 @@ SIGNATURES = {
     "amx_op_set_bc_operator": (I32, [P, I32]),
     "amx_op_finalize": (I32, [P]),
     "amx_op_get_info": (I32, [P, C.POINTER(OpInfo)]),
+    "amx_op_set_halo_transport": (I32, [P, HALO_FN, P, I32]),
     "amx_op_matvec": (I32, [P, P, P, I32]),
     "amx_op_matmult_mf": (I32, [P, P, P]),
     "amx_op_matvec_dev": (I32, [P, P, P, P]),
     "amx_op_matmult_dev": (I32, [P, P, P, P]),
     "amx_op_matvec_begin_dev": (I32, [P, P, P, I32, P]),
     "amx_op_matvec_end_dev": (I32, [P, P, P, I32, P]),
+    "amx_op_halo_pack_dev": (I32, [P, P, P, P]),
     "amx_op_halo_import_dev": (I32, [P, P, P, P]),
     "amx_op_halo_export_dev": (I32, [P, P, P, P]),
     "amx_op_halo_accumulate_dev": (I32, [P, P, P, P]),

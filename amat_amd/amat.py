@@ -11,7 +11,9 @@ from __future__ import annotations
 
 import ctypes as C
 import enum
-from typing import Sequence
+import sys
+import traceback
+from typing import Callable, Sequence
 
 import numpy as np
 
@@ -95,6 +97,16 @@ class BCOP(enum.IntEnum):
     PENALTY_DIAG = 2   # aMatBased::apply_bc_mat penalty branch
 
 
+class HALO(enum.IntEnum):
+    """Buffers handed to a halo transport (amx_halo_mode)."""
+    HOST = 0     # numpy views of pinned host buffers (an MPI caller)
+    DEVICE = 1   # device addresses and the stream (an RCCL caller)
+
+
+OWNER_TO_GHOST = 0   # ghost_receive_begin/end (include/aMatFree.hpp:1426-1549)
+GHOST_TO_OWNER = 1   # ghost_send_begin/end (:1552-1679)
+
+
 def _ptr(a) -> int | None:
     """Address of a contiguous numpy array or torch tensor (None for None)."""
     if a is None:
@@ -108,6 +120,33 @@ def _ptr(a) -> int | None:
             raise ValueError("tensor must be contiguous")
         return a.data_ptr()
     raise TypeError(f"unsupported array type {type(a)}")
+
+
+def _dev(t, n: int, name: str, device: int, dtype: str = "float64") -> int:
+    """Address of a contiguous torch tensor on HIP device `device` with dtype
+    `dtype` and at least n entries; raises before anything is launched."""
+    if t is None or not hasattr(t, "data_ptr"):
+        raise TypeError(f"{name} must be a torch tensor on a HIP device")
+    if not getattr(t, "is_cuda", False):
+        raise TypeError(f"{name} is on {t.device}; it must be on a HIP device")
+    if t.device.index is not None and t.device.index != device:
+        raise ValueError(f"{name} is on device {t.device.index}; the operator is on device {device}")
+    if str(t.dtype) != "torch." + dtype:
+        raise TypeError(f"{name} is {t.dtype}; expected torch.{dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if t.numel() < n:
+        raise ValueError(f"{name} has {t.numel()} entries; the operator needs {n}")
+    return t.data_ptr()
+
+
+def _host(a, n: int, name: str) -> int:
+    """Address of a contiguous float64 numpy array with at least n entries."""
+    if not isinstance(a, np.ndarray) or a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise TypeError(f"{name} must be a contiguous float64 numpy array")
+    if a.size < n:
+        raise ValueError(f"{name} has {a.size} entries; the operator needs {n}")
+    return a.ctypes.data
 
 
 def _arr(a, dtype) -> np.ndarray:
@@ -396,23 +435,31 @@ class aMatFree:
         return inf
 
     # -- host-array forms ----------------------------------------------
+    def _sizes(self):
+        inf = self.maps.info()
+        return inf.n_total, inf.n_owned, inf.n_pre_ghost + inf.n_post_ghost, inf.n_send
+
     def matvec(self, v: np.ndarray, u: np.ndarray, isGhosted: bool = False):
-        """aMatFree::matvec (include/aMatFree.hpp:1681-1775): v = K u."""
+        """aMatFree::matvec (include/aMatFree.hpp:1681-1775): v = K u; length
+        n_total when isGhosted, else n_owned.  More than one rank: with the
+        halo transport (set_halo_transport)."""
+        n_total, n_owned, _, _ = self._sizes()
+        n = n_total if isGhosted else n_owned
         uu = _arr(u, np.float64)
-        if v.dtype != np.float64 or not v.flags.c_contiguous:
-            raise ValueError("v must be a contiguous float64 array")
-        call("amx_op_matvec", self._h, _ptr(v), _ptr(uu), int(bool(isGhosted)))
+        pu, pv = _host(uu, n, "u"), _host(v, n, "v")
+        call("amx_op_matvec", self._h, pv, pu, int(bool(isGhosted)))
         return v
 
     def MatMult_mf(self, u: np.ndarray, out: np.ndarray | None = None) -> np.ndarray:
         """aMatFree::MatMult_mf (include/aMatFree.hpp:3893-3941) on owned-length arrays
         (`out`: a contiguous float64 array of u's length to write v into)."""
+        n_owned = self._sizes()[1]
         uu = _arr(u, np.float64)
         if out is None:
             out = np.zeros_like(uu)
         elif out.dtype != np.float64 or not out.flags.c_contiguous or out.shape != uu.shape:
             raise ValueError("out must be a contiguous float64 array of u's shape")
-        call("amx_op_matmult_mf", self._h, _ptr(out), _ptr(uu))
+        call("amx_op_matmult_mf", self._h, _host(out, n_owned, "v"), _host(uu, n_owned, "u"))
         return out
 
     def get_diagonal(self) -> np.ndarray:
@@ -424,39 +471,93 @@ class aMatFree:
     def apply_bc(self, rhs: np.ndarray) -> np.ndarray:
         """aMatFree::apply_bc -> apply_bc_rhs (include/aMatFree.hpp:957-971, 4536-4700)."""
         r = _arr(rhs, np.float64).copy()
-        call("amx_op_apply_bc_rhs", self._h, _ptr(r))
+        call("amx_op_apply_bc_rhs", self._h, _host(r, self._sizes()[1], "rhs"))
         return r
 
     # -- device forms (torch tensors) -----------------------------------
+    # vectors: float64 (float32 for the FP32 forms) tensors on the operator's
+    # device with at least n_total entries; checked before any launch
+    def _vec(self, t, name, dtype="float64"):
+        return _dev(t, self._sizes()[0], name, self.device, dtype)
+
     def matvec_dev(self, v, u, stream=None):
-        call("amx_op_matvec_dev", self._h, _ptr(v), _ptr(u), _stream_ptr(stream))
+        call("amx_op_matvec_dev", self._h, self._vec(v, "v"), self._vec(u, "u"), _stream_ptr(stream))
 
     def matmult_dev(self, v, u, stream=None):
-        call("amx_op_matmult_dev", self._h, _ptr(v), _ptr(u), _stream_ptr(stream))
+        call("amx_op_matmult_dev", self._h, self._vec(v, "v"), self._vec(u, "u"), _stream_ptr(stream))
 
     def matmult_f32_dev(self, precision: PREC, v, u, stream=None):
-        call("amx_op_matmult_f32_dev", self._h, int(precision), _ptr(v), _ptr(u), _stream_ptr(stream))
+        call("amx_op_matmult_f32_dev", self._h, int(precision), self._vec(v, "v", "float32"),
+             self._vec(u, "u", "float32"), _stream_ptr(stream))
 
     def matvec_begin_dev(self, v, u, apply_bc=True, stream=None):
-        call("amx_op_matvec_begin_dev", self._h, _ptr(v), _ptr(u), int(apply_bc), _stream_ptr(stream))
+        call("amx_op_matvec_begin_dev", self._h, self._vec(v, "v"), self._vec(u, "u"), int(apply_bc),
+             _stream_ptr(stream))
 
     def matvec_end_dev(self, v, u, apply_bc=True, stream=None):
-        call("amx_op_matvec_end_dev", self._h, _ptr(v), _ptr(u), int(apply_bc), _stream_ptr(stream))
+        call("amx_op_matvec_end_dev", self._h, self._vec(v, "v"), self._vec(u, "u"), int(apply_bc),
+             _stream_ptr(stream))
+
+    def halo_pack_dev(self, u, send, stream=None):
+        """ghost_receive_begin's pack (include/aMatFree.hpp:1452-1454): send[i] = u[send_ids[i]]."""
+        n_send = self._sizes()[3]
+        call("amx_op_halo_pack_dev", self._h, self._vec(u, "u"),
+             _dev(send, n_send, "send", self.device) if n_send else None, _stream_ptr(stream))
 
     def halo_import_dev(self, u, halo, stream=None):
-        call("amx_op_halo_import_dev", self._h, _ptr(u), _ptr(halo), _stream_ptr(stream))
+        n_ghost = self._sizes()[2]
+        call("amx_op_halo_import_dev", self._h, self._vec(u, "u"),
+             _dev(halo, n_ghost, "halo", self.device) if n_ghost else None, _stream_ptr(stream))
 
     def halo_export_dev(self, v, halo, stream=None):
-        call("amx_op_halo_export_dev", self._h, _ptr(v), _ptr(halo), _stream_ptr(stream))
+        n_ghost = self._sizes()[2]
+        call("amx_op_halo_export_dev", self._h, self._vec(v, "v"), _dev(halo, n_ghost, "halo", self.device),
+             _stream_ptr(stream))
 
     def halo_accumulate_dev(self, v, recv, stream=None):
-        call("amx_op_halo_accumulate_dev", self._h, _ptr(v), _ptr(recv), _stream_ptr(stream))
+        n_send = self._sizes()[3]
+        call("amx_op_halo_accumulate_dev", self._h, self._vec(v, "v"),
+             _dev(recv, n_send, "recv", self.device) if n_send else None, _stream_ptr(stream))
 
     def bc_fixup_dev(self, v, u, stream=None):
-        call("amx_op_bc_fixup_dev", self._h, _ptr(v), _ptr(u), _stream_ptr(stream))
+        call("amx_op_bc_fixup_dev", self._h, self._vec(v, "v"), self._vec(u, "u"), _stream_ptr(stream))
 
     def get_diagonal_dev(self, diag, stream=None):
-        call("amx_op_get_diagonal_dev", self._h, _ptr(diag), _stream_ptr(stream))
+        call("amx_op_get_diagonal_dev", self._h, self._vec(diag, "diag"), _stream_ptr(stream))
+
+    # -- halo transport -------------------------------------------------
+    def set_halo_transport(self, fn: Callable | None, mode: HALO = HALO.HOST):
+        """The exchange of a rank's share (amx_op_set_halo_transport):
+        fn(direction, send, recv, stream) moves send to the peers and fills
+        recv.  direction OWNER_TO_GHOST: send = u at the send ids (n_send),
+        recv = ghost values (n_pre + n_post, recv offsets); GHOST_TO_OWNER:
+        send = ghost slots of v (n_pre + n_post), recv = owners' contributions
+        (n_send, send offsets).  HOST: numpy views of pinned host buffers;
+        DEVICE: device addresses (ints) and the stream address.  None removes
+        the transport.  The call is collective: every rank makes it."""
+        if fn is None:
+            call("amx_op_set_halo_transport", self._h, _capi.HALO_FN(), None, int(mode))
+            self._halo_cb = None
+            return
+        _, _, n_ghost, n_send = self._sizes()
+        mode = HALO(mode)
+
+        def cb(ctx, direction, send, recv, stream):
+            try:
+                n_out, n_in = (n_send, n_ghost) if direction == OWNER_TO_GHOST else (n_ghost, n_send)
+                if mode == HALO.HOST:
+                    so = np.ctypeslib.as_array(send, shape=(n_out,)) if n_out else np.zeros(0)
+                    ri = np.ctypeslib.as_array(recv, shape=(n_in,)) if n_in else np.zeros(0)
+                    fn(direction, so, ri, stream)
+                else:
+                    fn(direction, C.cast(send, C.c_void_p).value, C.cast(recv, C.c_void_p).value, stream)
+                return 0
+            except Exception:  # reported through the calling entry point's status
+                traceback.print_exc(file=sys.stderr)
+                return 1
+
+        self._halo_cb = _capi.HALO_FN(cb)   # kept alive with the operator
+        call("amx_op_set_halo_transport", self._h, self._halo_cb, None, int(mode))
 
     def get_timer(self) -> tuple:
         """aMatFree::get_timer (include/aMatFree.hpp:239-245): seconds
@@ -471,8 +572,8 @@ class aMatFree:
         """Mean ms of the element kernels, the gather and the whole application
         over n_iter runs (hipEvents on the launch stream)."""
         a, b, c = C.c_double(), C.c_double(), C.c_double()
-        call("amx_op_time_kernels_dev", self._h, int(n_iter), _ptr(v), _ptr(u), C.byref(a), C.byref(b), C.byref(c),
-             _stream_ptr(stream))
+        call("amx_op_time_kernels_dev", self._h, int(n_iter), self._vec(v, "v"), self._vec(u, "u"), C.byref(a),
+             C.byref(b), C.byref(c), _stream_ptr(stream))
         return a.value, b.value, c.value
 
     def stream(self) -> int:
@@ -484,14 +585,15 @@ class aMatFree:
     # -- solvers ---------------------------------------------------------
     def cg(self, b, x, rtol=1e-10, max_it=10000, stream=None) -> _capi.SolveStats:
         st = _capi.SolveStats()
-        call("amx_cg_solve_dev", self._h, _ptr(b), _ptr(x), float(rtol), int(max_it), C.byref(st),
-             _stream_ptr(stream))
+        call("amx_cg_solve_dev", self._h, self._vec(b, "b"), self._vec(x, "x"), float(rtol), int(max_it),
+             C.byref(st), _stream_ptr(stream))
         return st
 
     def ir_cg(self, precision: PREC, b, x, rtol=1e-10, inner_rtol=1e-2, max_outer=50, max_inner=10000,
               stream=None) -> _capi.SolveStats:
         st = _capi.SolveStats()
-        call("amx_ir_cg_solve_dev", self._h, int(precision), _ptr(b), _ptr(x), float(rtol), float(inner_rtol),
+        call("amx_ir_cg_solve_dev", self._h, int(precision), self._vec(b, "b"), self._vec(x, "x"), float(rtol),
+             float(inner_rtol),
              int(max_outer), int(max_inner), C.byref(st), _stream_ptr(stream))
         return st
 
@@ -563,8 +665,8 @@ def version() -> str:
     return _capi.lib().amx_version().decode()
 
 
-__all__ = ["AmatError", "BC_METH", "BCOP", "Error", "MATFREE_TYPE", "Maps", "MultiRHS", "PREC", "SCATTER",
-           "aMatFree", "device_count", "version"]
+__all__ = ["AmatError", "BC_METH", "BCOP", "Error", "GHOST_TO_OWNER", "HALO", "MATFREE_TYPE", "Maps", "MultiRHS",
+           "OWNER_TO_GHOST", "PREC", "SCATTER", "aMatFree", "device_count", "version"]
 
 
 def flatten_rows(rows: Sequence) -> tuple[np.ndarray, np.ndarray]:

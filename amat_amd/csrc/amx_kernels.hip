@@ -537,6 +537,13 @@ __global__ void set_entries_kernel(double* __restrict__ u, const uint32_t* __res
     if (i < n) u[ids[i]] = value;
 }
 
+__global__ void pack_kernel(double* __restrict__ dst, const double* __restrict__ src,
+                            const uint32_t* __restrict__ ids, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[ids[i]];
+}
+
 __global__ void halo_accumulate_kernel(double* __restrict__ v, const double* __restrict__ recv,
                                        const uint32_t* __restrict__ dst, const uint32_t* __restrict__ off,
                                        const uint32_t* __restrict__ src, uint32_t n_unique)
@@ -813,6 +820,13 @@ void launch_set_entries(double* u, const uint32_t* ids, uint32_t n, double value
 {
     if (n == 0) return;
     set_entries_kernel<<<blocks_for(n, 256), 256, 0, s>>>(u, ids, n, value);
+    AMX_HIP(hipGetLastError());
+}
+
+void launch_pack(double* dst, const double* src, const uint32_t* ids, uint32_t n, hipStream_t s)
+{
+    if (n == 0) return;
+    pack_kernel<<<blocks_for(n, 256), 256, 0, s>>>(dst, src, ids, n);
     AMX_HIP(hipGetLastError());
 }
 

@@ -124,6 +124,8 @@ void launch_fill(void* p, uint64_t n, bool vt_double, double value, hipStream_t 
 void launch_coef(const double* diag, const uint32_t* cdofs, uint32_t nc, double penalty, double* coef64,
                  hipStream_t s);
 void launch_set_entries(double* u, const uint32_t* ids, uint32_t n, double value, hipStream_t s);
+// dst[i] = src[ids[i]], i < n (the owner's halo pack)
+void launch_pack(double* dst, const double* src, const uint32_t* ids, uint32_t n, hipStream_t s);
 // halo accumulate: v[dst[i]] += sum_{q in [off[i], off[i+1])} recv[src[q]], i < n_unique
 void launch_halo_accumulate(double* v, const double* recv, const uint32_t* dst, const uint32_t* off,
                             const uint32_t* src, uint32_t n_unique, hipStream_t s);

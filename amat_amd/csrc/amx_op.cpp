@@ -96,6 +96,73 @@ FreeArgs free_args(const Op& op, const Group& g, const void* u, void* out, uint3
     return a;
 }
 
+// ---- halo exchange through the caller's transport (ghost_receive_begin/end
+// and ghost_send_begin/end, include/aMatFree.hpp:1426-1679)
+
+// hx_send holds n_out values; the transport fills n_in values of hx_recv
+void transport(Op& op, int dir, uint32_t n_out, uint32_t n_in, hipStream_t s)
+{
+    AMX_REQUIRE(op.halo_fn != nullptr, AMX_NOT_IMPLEMENTED, "no halo transport registered");
+    int rc = 0;
+    if (op.halo_mode == AMX_HALO_HOST) {
+        if (n_out) AMX_HIP(hipMemcpyAsync(op.hh_send, op.hx_send.p, n_out * sizeof(double), hipMemcpyDeviceToHost, s));
+        // also orders the previous exchange's copy out of hh_recv before fn writes it
+        AMX_HIP(hipStreamSynchronize(s));
+        rc = op.halo_fn(op.halo_ctx, dir, op.hh_send, op.hh_recv, s);
+        if (rc == 0 && n_in)
+            AMX_HIP(hipMemcpyAsync(op.hx_recv.p, op.hh_recv, n_in * sizeof(double), hipMemcpyHostToDevice, s));
+    } else {
+        rc = op.halo_fn(op.halo_ctx, dir, op.hx_send.as<double>(), op.hx_recv.as<double>(), s);
+    }
+    AMX_REQUIRE(rc == 0, AMX_WRONG_COMMUNICATION, "halo transport returned " + std::to_string(rc));
+}
+
+// owners' values into the ghost slots of u (pack, transport, import)
+void exchange_u(Op& op, double* u, hipStream_t s)
+{
+    launch_pack(op.hx_send.as<double>(), u, op.halo_send_ids.as<uint32_t>(), op.n_send(), s);
+    transport(op, AMX_HALO_OWNER_TO_GHOST, op.n_send(), op.n_ghost(), s);
+    const double* r = op.hx_recv.as<double>();
+    if (op.n_pre) AMX_HIP(hipMemcpyAsync(u, r, op.n_pre * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (op.n_post)
+        AMX_HIP(hipMemcpyAsync(u + op.n_pre + op.n_owned, r + op.n_pre, op.n_post * sizeof(double),
+                               hipMemcpyDeviceToDevice, s));
+}
+
+// ghost slots of v added into their owners' rows (export, transport, accumulate)
+void exchange_v(Op& op, double* v, hipStream_t s)
+{
+    double* x = op.hx_send.as<double>();
+    if (op.n_pre) AMX_HIP(hipMemcpyAsync(x, v, op.n_pre * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (op.n_post)
+        AMX_HIP(hipMemcpyAsync(x + op.n_pre, v + op.n_pre + op.n_owned, op.n_post * sizeof(double),
+                               hipMemcpyDeviceToDevice, s));
+    transport(op, AMX_HALO_GHOST_TO_OWNER, op.n_ghost(), op.n_send(), s);
+    launch_halo_accumulate(v, op.hx_recv.as<double>(), op.halo_dst.as<uint32_t>(), op.halo_off.as<uint32_t>(),
+                           op.halo_src.as<uint32_t>(), op.n_halo_unique, s);
+}
+
+// the assembled diagonal on the owned rows (mat_get_diagonal_ghosted's
+// ghost_send_begin/end, :1051-1053) and the penalty coefficients from it
+void assemble_diag(Op& op, hipStream_t s)
+{
+    exchange_v(op, op.diag.as<double>(), s);
+    launch_fill(op.coef64.p, op.n_total, true, 0.0, s);
+    launch_coef(op.diag.as<double>(), op.cdofs.as<uint32_t>(), op.n_cdofs, op.penalty(), op.coef64.as<double>(), s);
+    AMX_HIP(hipStreamSynchronize(s));
+    op.diag_assembled = true;
+}
+
+// more than one rank: the exchanging forms need the transport
+void require_exchange(const Op& op)
+{
+    if (op.size == 1) return;
+    AMX_REQUIRE(op.halo_fn != nullptr, AMX_NOT_IMPLEMENTED,
+                "a rank's share exchanges halos: register amx_op_set_halo_transport, or drive "
+                "amx_op_matvec_begin_dev / halo_* / matvec_end_dev / bc_fixup_dev");
+    AMX_REQUIRE(op.diag_assembled, AMX_INVALID_STATE, "diagonal not assembled across ranks");
+}
+
 // Build slots, device K, maps, transposed lists and BC tables.
 void finalize_op(Op& op)
 {
@@ -340,6 +407,8 @@ void finalize_op(Op& op)
         AMX_HIP(hipStreamSynchronize(s));  // host vectors leave scope
     }
     AMX_HIP(hipStreamSynchronize(s));
+    op.diag_assembled = op.size == 1;
+    if (op.size > 1 && op.halo_fn) assemble_diag(op, s);
     op.t_scatter = op.t_mv = op.t_gather = op.t_total = 0.0;
     op.finalized = true;
 }
@@ -463,6 +532,28 @@ void apply(Op& op, KType kt, bool vt_double, int bc, const void* u, void* v, hip
     }
 }
 
+// v = A u with BC form bc (FP64) on the handle's share: one rank as apply();
+// several ranks with the halo exchanges around the dependent elements, the
+// ghost accumulation, then the constrained rows (matvec_ghosted_OMP's
+// ghost_receive / ghost_send bracketing, :1795, 2016, 2234-2235)
+void apply_full(Op& op, int bc, double* u, double* v, hipStream_t s)
+{
+    if (op.size == 1) {
+        apply(op, KType::F64, true, bc, u, v, s);
+        return;
+    }
+    require_exchange(op);
+    const bool masked = bc == AMX_BCOP_IDENTITY;
+    const bool atomic = op.scatter == AMX_SCATTER_ATOMIC;
+    if (atomic) launch_fill(v, op.n_total, true, 0.0, s);
+    ebe_launch(op, KType::F64, true, masked, 0, u, v, s);
+    exchange_u(op, u, s);
+    ebe_launch(op, KType::F64, true, masked, 1, u, v, s);
+    if (!atomic) gather_launch(op, true, AMX_BCOP_NONE, u, v, s);
+    exchange_v(op, v, s);
+    launch_bc_fixup(true, bc, op.cdofs.as<uint32_t>(), op.n_cdofs, op.coef64.as<double>(), u, v, s);
+}
+
 }  // namespace amx
 
 using amx::DeviceGuard;
@@ -508,7 +599,8 @@ int amx_op_create(const amx_maps* maps, int bc_meth, int device, amx_op** out)
             }
             for (uint64_t gid : maps->owned_free) op.owned_free_local.push_back((uint32_t)(gid - op.dof_lo) + op.n_pre);
         }
-        if (maps->size > 1 && maps->has_peer_lists) op.send_ids = maps->send_ids;
+        op.has_peer_lists = maps->size > 1 && maps->has_peer_lists;
+        if (op.has_peer_lists) op.send_ids = maps->send_ids;
         // element groups by DoF count
         op.elem_group.assign(op.n_elem, -1);
         op.elem_index.assign(op.n_elem, -1);
@@ -732,14 +824,48 @@ int amx_op_get_info(const amx_op* h, amx_op_info* info)
     });
 }
 
+int amx_op_set_halo_transport(amx_op* h, amx_halo_fn fn, void* ctx, int mode)
+{
+    return guarded(__func__, [&] {
+        AMX_REQUIRE(h, AMX_INVALID_ARGUMENT, "op is NULL");
+        Op& op = h->op;
+        if (!fn) {
+            op.halo_fn = nullptr;
+            op.halo_ctx = nullptr;
+            return;
+        }
+        AMX_REQUIRE(mode == AMX_HALO_HOST || mode == AMX_HALO_DEVICE, AMX_INVALID_ARGUMENT, "unknown transport mode");
+        AMX_REQUIRE(op.size > 1, AMX_INVALID_STATE, "one rank has no halo to exchange");
+        AMX_REQUIRE(op.has_peer_lists, AMX_INVALID_STATE, "maps: peer requests have not been set");
+        DeviceGuard dg(op.device);
+        const size_t b = std::max<uint32_t>(std::max(op.n_send(), op.n_ghost()), 1) * sizeof(double);
+        if (!op.halo_send_ids.p) amx::upload(op.halo_send_ids, op.send_ids, op.stream, op.device_bytes);
+        if (op.hx_send.bytes < b) {
+            op.hx_send.alloc(b);
+            op.hx_recv.alloc(b);
+            op.device_bytes += 2 * b;
+        }
+        if (mode == AMX_HALO_HOST && !op.hh_send) {
+            AMX_HIP(hipHostMalloc((void**)&op.hh_send, b, hipHostMallocDefault));
+            AMX_HIP(hipHostMalloc((void**)&op.hh_recv, b, hipHostMallocDefault));
+        }
+        AMX_HIP(hipStreamSynchronize(op.stream));
+        op.halo_fn = fn;
+        op.halo_ctx = ctx;
+        op.halo_mode = mode;
+        if (op.finalized && !op.diag_assembled) amx::assemble_diag(op, op.stream);
+    });
+}
+
 int amx_op_matvec_dev(amx_op* h, double* v, const double* u, void* stream)
 {
     return guarded(__func__, [&] {
         AMX_REQUIRE(h && v && u, AMX_INVALID_ARGUMENT, "NULL argument");
         Op& op = h->op;
         amx::require_finalized(op);
+        amx::require_exchange(op);
         DeviceGuard dg(op.device);
-        amx::apply(op, KType::F64, true, AMX_BCOP_NONE, u, v, op.pick(stream));
+        amx::apply_full(op, AMX_BCOP_NONE, const_cast<double*>(u), v, op.pick(stream));
     });
 }
 
@@ -749,16 +875,9 @@ int amx_op_matmult_dev(amx_op* h, double* v, const double* u, void* stream)
         AMX_REQUIRE(h && v && u, AMX_INVALID_ARGUMENT, "NULL argument");
         Op& op = h->op;
         amx::require_finalized(op);
+        amx::require_exchange(op);
         DeviceGuard dg(op.device);
-        if (op.size > 1) {
-            // several ranks: constrained rows are fixed after the halo
-            // accumulation (amx_op_bc_fixup_dev)
-            hipStream_t s = op.pick(stream);
-            amx::ebe_launch(op, KType::F64, true, op.bc_operator == AMX_BCOP_IDENTITY, 2, u, v, s);
-            amx::gather_launch(op, true, 0, u, v, s);
-        } else {
-            amx::apply(op, KType::F64, true, op.bc_operator, u, v, op.pick(stream));
-        }
+        amx::apply_full(op, op.bc_operator, const_cast<double*>(u), v, op.pick(stream));
     });
 }
 
@@ -783,9 +902,10 @@ int amx_op_matvec_begin_dev(amx_op* h, double* v, const double* u, int apply_bc,
         AMX_REQUIRE(h && v && u, AMX_INVALID_ARGUMENT, "NULL argument");
         Op& op = h->op;
         amx::require_finalized(op);
-        AMX_REQUIRE(op.scatter == AMX_SCATTER_DETERMINISTIC, AMX_NOT_IMPLEMENTED, "two-phase launch uses the deterministic gather");
         DeviceGuard dg(op.device);
-        amx::ebe_launch(op, KType::F64, true, apply_bc && op.bc_operator == AMX_BCOP_IDENTITY, 0, u, v, op.pick(stream));
+        hipStream_t s = op.pick(stream);
+        if (op.scatter == AMX_SCATTER_ATOMIC) amx::launch_fill(v, op.n_total, true, 0.0, s);
+        amx::ebe_launch(op, KType::F64, true, apply_bc && op.bc_operator == AMX_BCOP_IDENTITY, 0, u, v, s);
     });
 }
 
@@ -795,14 +915,36 @@ int amx_op_matvec_end_dev(amx_op* h, double* v, const double* u, int apply_bc, v
         AMX_REQUIRE(h && v && u, AMX_INVALID_ARGUMENT, "NULL argument");
         Op& op = h->op;
         amx::require_finalized(op);
-        AMX_REQUIRE(op.scatter == AMX_SCATTER_DETERMINISTIC, AMX_NOT_IMPLEMENTED, "two-phase launch uses the deterministic gather");
         DeviceGuard dg(op.device);
         hipStream_t s = op.pick(stream);
         const bool masked = apply_bc && op.bc_operator == AMX_BCOP_IDENTITY;
         amx::ebe_launch(op, KType::F64, true, masked, 1, u, v, s);
-        // one rank: BC rows fused; several: after accumulation (bc_fixup)
+        // one rank: BC rows now (fused into the gather); several: after the
+        // accumulation (bc_fixup)
         const int bc = (apply_bc && op.size == 1) ? op.bc_operator : AMX_BCOP_NONE;
-        amx::gather_launch(op, true, bc, u, v, s);
+        if (op.scatter == AMX_SCATTER_ATOMIC)
+            amx::launch_bc_fixup(true, bc, op.cdofs.as<uint32_t>(), op.n_cdofs, op.coef64.as<double>(), u, v, s);
+        else
+            amx::gather_launch(op, true, bc, u, v, s);
+    });
+}
+
+int amx_op_halo_pack_dev(amx_op* h, const double* u, double* send, void* stream)
+{
+    return guarded(__func__, [&] {
+        AMX_REQUIRE(h && u, AMX_INVALID_ARGUMENT, "NULL argument");
+        Op& op = h->op;
+        amx::require_finalized(op);
+        AMX_REQUIRE(op.size == 1 || op.has_peer_lists, AMX_INVALID_STATE, "maps: peer requests have not been set");
+        AMX_REQUIRE(send || op.n_send() == 0, AMX_INVALID_ARGUMENT, "send is NULL");
+        if (op.n_send() == 0) return;
+        DeviceGuard dg(op.device);
+        hipStream_t s = op.pick(stream);
+        if (!op.halo_send_ids.p) {
+            amx::upload(op.halo_send_ids, op.send_ids, op.stream, op.device_bytes);
+            AMX_HIP(hipStreamSynchronize(op.stream));
+        }
+        amx::launch_pack(send, u, op.halo_send_ids.as<uint32_t>(), op.n_send(), s);
     });
 }
 
@@ -812,6 +954,7 @@ int amx_op_halo_import_dev(amx_op* h, double* u, const double* halo, void* strea
         AMX_REQUIRE(h && u, AMX_INVALID_ARGUMENT, "NULL argument");
         Op& op = h->op;
         amx::require_finalized(op);
+        AMX_REQUIRE(halo || op.n_ghost() == 0, AMX_INVALID_ARGUMENT, "halo is NULL");
         DeviceGuard dg(op.device);
         hipStream_t s = op.pick(stream);
         if (op.n_pre) AMX_HIP(hipMemcpyAsync(u, halo, op.n_pre * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -844,6 +987,7 @@ int amx_op_halo_accumulate_dev(amx_op* h, double* v, const double* recv, void* s
         AMX_REQUIRE(h && v, AMX_INVALID_ARGUMENT, "NULL argument");
         Op& op = h->op;
         amx::require_finalized(op);
+        AMX_REQUIRE(recv || op.n_halo_unique == 0, AMX_INVALID_ARGUMENT, "recv is NULL");
         DeviceGuard dg(op.device);
         amx::launch_halo_accumulate(v, recv, op.halo_dst.as<uint32_t>(), op.halo_off.as<uint32_t>(),
                                     op.halo_src.as<uint32_t>(), op.n_halo_unique, op.pick(stream));
@@ -868,6 +1012,7 @@ int amx_op_matvec(amx_op* h, double* v, const double* u, int is_ghosted)
         AMX_REQUIRE(h && v && u, AMX_INVALID_ARGUMENT, "NULL argument");
         Op& op = h->op;
         amx::require_finalized(op);
+        amx::require_exchange(op);
         DeviceGuard dg(op.device);
         amx::ensure_scratch(op);
         hipStream_t s = op.stream;
@@ -878,11 +1023,11 @@ int amx_op_matvec(amx_op* h, double* v, const double* u, int is_ghosted)
         if (is_ghosted) {
             AMX_HIP(hipMemcpyAsync(du, u, op.n_total * sizeof(double), hipMemcpyHostToDevice, s));
         } else {  // local_to_ghost (:824-838): ghost slots are 0
-            amx::launch_fill(du, op.n_total, true, 0.0, s);
+            if (op.n_ghost()) amx::launch_fill(du, op.n_total, true, 0.0, s);
             AMX_HIP(hipMemcpyAsync(du + op.n_pre, u, op.n_owned * sizeof(double), hipMemcpyHostToDevice, s));
         }
         tm.mark(1, s);
-        amx::apply(op, KType::F64, true, AMX_BCOP_NONE, du, dv, s);
+        amx::apply_full(op, AMX_BCOP_NONE, du, dv, s);
         tm.mark(2, s);
         if (is_ghosted) AMX_HIP(hipMemcpyAsync(v, dv, op.n_total * sizeof(double), hipMemcpyDeviceToHost, s));
         else AMX_HIP(hipMemcpyAsync(v, dv + op.n_pre, op.n_owned * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -898,17 +1043,21 @@ int amx_op_matmult_mf(amx_op* h, double* vv, const double* uu)
         AMX_REQUIRE(h && vv && uu, AMX_INVALID_ARGUMENT, "NULL argument");
         Op& op = h->op;
         amx::require_finalized(op);
-        AMX_REQUIRE(op.size == 1, AMX_NOT_IMPLEMENTED, "host-array MatMult_mf is one-rank; use the device halo API");
+        amx::require_exchange(op);
         DeviceGuard dg(op.device);
         amx::ensure_scratch(op);
         hipStream_t s = op.stream;
+        double* du = op.scratch_u.as<double>();
+        double* dv = op.scratch_v.as<double>();
         amx::HostCallTimer tm(op);
         tm.mark(0, s);
-        AMX_HIP(hipMemcpyAsync(op.scratch_u.p, uu, op.n_owned * sizeof(double), hipMemcpyHostToDevice, s));
+        // local_to_ghost (:3913): the ghost slots are filled by the exchange
+        if (op.n_ghost()) amx::launch_fill(du, op.n_total, true, 0.0, s);
+        AMX_HIP(hipMemcpyAsync(du + op.n_pre, uu, op.n_owned * sizeof(double), hipMemcpyHostToDevice, s));
         tm.mark(1, s);
-        amx::apply(op, KType::F64, true, op.bc_operator, op.scratch_u.p, op.scratch_v.p, s);
+        amx::apply_full(op, op.bc_operator, du, dv, s);
         tm.mark(2, s);
-        AMX_HIP(hipMemcpyAsync(vv, op.scratch_v.p, op.n_owned * sizeof(double), hipMemcpyDeviceToHost, s));
+        AMX_HIP(hipMemcpyAsync(vv, dv + op.n_pre, op.n_owned * sizeof(double), hipMemcpyDeviceToHost, s));
         tm.mark(3, s);
         AMX_HIP(hipStreamSynchronize(s));
         tm.finish();
@@ -934,6 +1083,7 @@ int amx_op_get_diagonal_dev(amx_op* h, double* diag, void* stream)
         AMX_REQUIRE(h && diag, AMX_INVALID_ARGUMENT, "NULL argument");
         Op& op = h->op;
         amx::require_finalized(op);
+        amx::require_exchange(op);  // the owned rows hold the assembled diagonal
         DeviceGuard dg(op.device);
         hipStream_t s = op.pick(stream);
         AMX_HIP(hipMemcpyAsync(diag, op.diag.p, op.n_total * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -965,11 +1115,11 @@ int amx_op_apply_bc_rhs(amx_op* h, double* rhs)
         AMX_REQUIRE(h && rhs, AMX_INVALID_ARGUMENT, "NULL argument");
         Op& op = h->op;
         amx::require_finalized(op);
-        AMX_REQUIRE(op.size == 1, AMX_NOT_IMPLEMENTED, "apply_bc_rhs is one-rank");
         AMX_REQUIRE(op.has_bdr, AMX_INVALID_STATE, "no boundary map");
         DeviceGuard dg(op.device);
         amx::ensure_scratch(op);
         if (op.bc_meth == AMX_BC_IMATRIX) {
+            amx::require_exchange(op);
             // K_fc U_c: the operator on the vector holding the prescribed values
             // at constrained DoFs (aMatBased::apply_bc_rhs :364-400 form of
             // aMatFree::apply_bc_rhs :4556-4648)
@@ -979,7 +1129,11 @@ int amx_op_apply_bc_rhs(amx_op* h, double* rhs)
             std::vector<double> w(op.n_total);
             hipStream_t s = op.stream;
             AMX_HIP(hipMemcpyAsync(op.scratch_u.p, uc.data(), op.n_total * sizeof(double), hipMemcpyHostToDevice, s));
+            // the product of this rank's elements; the ghost rows reach their
+            // owners (the reference's ADD_VALUES into KfcUcVec, :4640-4646);
+            // uc keeps this rank's prescribed values in its ghost slots
             amx::apply(op, KType::F64, true, AMX_BCOP_NONE, op.scratch_u.p, op.scratch_v.p, s);
+            if (op.size > 1) amx::exchange_v(op, op.scratch_v.as<double>(), s);
             AMX_HIP(hipMemcpyAsync(w.data(), op.scratch_v.p, op.n_total * sizeof(double), hipMemcpyDeviceToHost, s));
             AMX_HIP(hipStreamSynchronize(s));
             for (uint32_t d : op.owned_free_local) rhs[d - op.n_pre] += -1.0 * w[d];

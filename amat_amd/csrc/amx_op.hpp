@@ -65,6 +65,7 @@ struct Op {
     std::vector<double> owned_c_vals;
     std::vector<uint32_t> owned_free_local;
     std::vector<uint32_t> send_ids;      // ghost_send_end targets (concatenated over ranks)
+    bool has_peer_lists = false;
 
     // element matrices
     std::vector<int32_t> elem_group, elem_index;
@@ -86,6 +87,18 @@ struct Op {
     uint32_t n_cdofs = 0, n_diag_bc = 0, n_ghost_c = 0;
     DevBuf halo_dst, halo_off, halo_src;
     uint32_t n_halo_unique = 0;
+    // caller's halo transport (amx_op_set_halo_transport) and its staging:
+    // device send ids, device buffers of max(n_send, n_ghost) values, pinned
+    // host copies for AMX_HALO_HOST
+    amx_halo_fn halo_fn = nullptr;
+    void* halo_ctx = nullptr;
+    int halo_mode = AMX_HALO_HOST;
+    bool diag_assembled = false;         // ghost rows of diag added into their owners
+    DevBuf halo_send_ids, hx_send, hx_recv;
+    double* hh_send = nullptr;
+    double* hh_recv = nullptr;
+    uint32_t n_ghost() const { return n_pre + n_post; }
+    uint32_t n_send() const { return (uint32_t)send_ids.size(); }
     DevBuf scratch_u, scratch_v;         // host-array API staging
     // host-array matvec timers (aMatFree::get_timer): events of the phases
     // (created with the staging buffers) and seconds accumulated since finalize
@@ -103,6 +116,8 @@ struct Op {
     ~Op()
     {
         if (host_sc) (void)hipHostFree(host_sc);
+        if (hh_send) (void)hipHostFree(hh_send);
+        if (hh_recv) (void)hipHostFree(hh_recv);
         for (hipEvent_t e : tev)
             if (e) (void)hipEventDestroy(e);
     }

@@ -42,10 +42,17 @@ __device__ __forceinline__ double block_sum(double x, double* sh)
 // No device-scope fence: on this chip a release/acquire fence at agent scope
 // writes back / invalidates the whole L2 of the XCD (buffer_wbl2 / buffer_inv),
 // which, once per block, dominated the solver kernels.  Instead the partials
-// are deposited with agent-scope atomic exchanges (performed at the point of
-// coherence of all XCDs) whose completion is awaited before a counter is
-// incremented, and the finishing block reads them with agent-scope atomic
-// loads.
+// are deposited with agent-scope atomic exchanges and the finishing block
+// reads them with agent-scope atomic loads; both are performed at the point
+// of coherence of all XCDs, not in an XCD's L2.  The ordering argument: the
+// exchanges return the previous values (their results feed an asm statement,
+// so the compiler must emit returning atomics and wait for their data), and a
+// returning atomic's data comes back only after the operation has been
+// performed at that point; the counter increment is issued after that wait
+// (the asm statement clobbers memory, so nothing moves above it).  Whoever
+// sees the counter at its final value therefore finds every deposit already
+// performed.  This rests on the hardware's atomics, not on the HIP memory
+// model's fences: relaxed atomics carry no ordering there.
 //
 // Two levels of counters: atomics on one address serialize at the point of
 // coherence, so the blocks count per group of kGroup (the group's last block
@@ -67,14 +74,14 @@ __device__ bool reduce_partials_n(const double (&v)[NV], double* partials, uint3
     double* gpart = partials + kGroupOff;
     uint32_t* gcount = reinterpret_cast<uint32_t*>(partials + kCntOff);
     if (threadIdx.x == 0) {
-        double old = 0.0;
+        double old[NV];
         for (int k = 0; k < NV; k++)
-            old += __hip_atomic_exchange(&partials[(size_t)k * nblk + blk], b[k], __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-        // every exchange has returned (its value is used) before the counter moves
-        const uint32_t one = old == old ? 1u : 1u;
+            old[k] = __hip_atomic_exchange(&partials[(size_t)k * nblk + blk], b[k], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+        // every exchange has returned before the counter moves
+        for (int k = 0; k < NV; k++) __asm__ volatile("" ::"v"(old[k]) : "memory");
         __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        const uint32_t prev = __hip_atomic_fetch_add(&gcount[g], one, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t prev = __hip_atomic_fetch_add(&gcount[g], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         stage = (prev == gsize - 1) ? 1u : 0u;
     }
     __syncthreads();
@@ -89,14 +96,14 @@ __device__ bool reduce_partials_n(const double (&v)[NV], double* partials, uint3
         gs[k] = block_sum(x, sh);
     }
     if (threadIdx.x == 0) {
-        double old = 0.0;
+        double old[NV];
         for (int k = 0; k < NV; k++)
-            old += __hip_atomic_exchange(&gpart[(size_t)k * kGroup + g], gs[k], __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
+            old[k] = __hip_atomic_exchange(&gpart[(size_t)k * kGroup + g], gs[k], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&gcount[g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t one = old == old ? 1u : 1u;
+        for (int k = 0; k < NV; k++) __asm__ volatile("" ::"v"(old[k]) : "memory");
         __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        const uint32_t prev = __hip_atomic_fetch_add(top, one, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t prev = __hip_atomic_fetch_add(top, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         stage = (prev == n_groups - 1) ? 2u : 0u;
     }
     __syncthreads();

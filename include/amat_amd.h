@@ -267,29 +267,81 @@ int amx_op_set_bc_operator(amx_op* op, int bc_operator);
 int amx_op_finalize(amx_op* op);
 int amx_op_get_info(const amx_op* op, amx_op_info* info);
 
+/* ------------------------------------------------------- halo transport
+ * The reference's matvec on a rank's share is bracketed by MPI point-to-point
+ * exchanges through the maps' send/recv lists: ghost_receive_begin/end
+ * (include/aMatFree.hpp:1426-1549) copies owners' u into the ghost slots
+ * before the dependent elements, ghost_send_begin/end (:1552-1679) adds the
+ * ghost slots of v into their owners afterwards.  This library holds no
+ * communicator: the caller registers the transport that moves the buffers
+ * (MPI, RCCL, or threads sharing one process), and the library packs,
+ * imports and accumulates around it.
+ *
+ * fn(ctx, direction, send, recv, stream) is called once per exchange with
+ *   AMX_HALO_OWNER_TO_GHOST: send[n_send] = u at the send ids (peer i's part
+ *     at send offset i, count i of amx_maps_get_send_lists); recv[n_pre +
+ *     n_post] receives the ghost values, peer i's part at recv offset i,
+ *     count i (amx_maps_get_recv_lists) -- ghost_receive_begin's layout;
+ *   AMX_HALO_GHOST_TO_OWNER: send[n_pre + n_post] = the ghost slots of v
+ *     (pre then post), peer i's part at recv offset i; recv[n_send] receives
+ *     the owners' contributions, peer i's part at send offset i --
+ *     ghost_send_begin's layout.
+ * mode AMX_HALO_HOST: send/recv are pinned host buffers; the library has
+ * synchronized its stream before the call and copies recv back after it (an
+ * MPI caller, as the reference).  AMX_HALO_DEVICE: device buffers; the
+ * transfers must be complete, or ordered on `stream` (an RCCL caller), when
+ * fn returns.  fn returns 0 on success; anything else fails the calling
+ * entry point with AMX_WRONG_COMMUNICATION.  Every rank must make the same
+ * sequence of exchanging calls, as with the reference's MPI calls. */
+enum amx_halo_direction { AMX_HALO_OWNER_TO_GHOST = 0, AMX_HALO_GHOST_TO_OWNER = 1 };
+enum amx_halo_mode { AMX_HALO_HOST = 0, AMX_HALO_DEVICE = 1 };
+typedef int (*amx_halo_fn)(void* ctx, int direction, const double* send, double* recv, void* stream);
+
+/* Registers (fn != NULL) or removes (fn == NULL) the transport of a handle
+ * whose maps have more than one rank and peer lists.  The first exchange with
+ * the handle's finalized state adds the ghost rows of the element diagonals
+ * into their owners (mat_get_diagonal_ghosted's ghost_send, :1051-1053), so
+ * that the diagonal, the penalty coefficients p - K_cc and the Jacobi
+ * diagonal are the assembled ones: this happens inside finalize when a
+ * transport is registered, else inside this call -- on every rank. */
+int amx_op_set_halo_transport(amx_op* op, amx_halo_fn fn, void* ctx, int mode);
+
 /* aMatFree::matvec (include/aMatFree.hpp:1681-1775): v = K u, no BCs, host
- * arrays.  is_ghosted != 0: length n_total; else n_owned (ghost slots 0). */
+ * arrays.  is_ghosted != 0: length n_total; else n_owned (local_to_ghost /
+ * ghost_to_local, :824-852: ghost slots start at 0).  More than one rank:
+ * needs the halo transport (AMX_NOT_IMPLEMENTED without one); the ghost slots
+ * of u are filled from their owners and owners' rows receive the ghost
+ * contributions, as the reference's exchanges do. */
 int amx_op_matvec(amx_op* op, double* v, const double* u, int is_ghosted);
 
 /* aMatFree::MatMult_mf array semantics (include/aMatFree.hpp:3893-3941) with
- * the configured BC form, owned-length host arrays (one rank). */
+ * the configured BC form, owned-length host arrays.  More than one rank: with
+ * the halo transport, as amx_op_matvec. */
 int amx_op_matmult_mf(amx_op* op, double* vv, const double* uu);
 
 /* Device forms on ghosted-length device arrays, asynchronous on `stream`.
  * matvec_dev: v = K u.  matmult_dev: the configured BC form.  One rank:
- * ghost slots are empty.  Several ranks: see amx_op_matvec_begin_dev. */
+ * ghost slots are empty.  More than one rank: the whole exchange runs through
+ * the halo transport (the ghost slots of u are overwritten with the owners'
+ * values, as ghost_receive_end does); without one they return
+ * AMX_NOT_IMPLEMENTED -- the caller then drives the pieces itself
+ * (amx_op_matvec_begin_dev and the halo entry points below). */
 int amx_op_matvec_dev(amx_op* op, double* v, const double* u, void* stream);
 int amx_op_matmult_dev(amx_op* op, double* v, const double* u, void* stream);
 
 /* Two-phase form of matvec_ghosted_gpuOverGpu (include/aMatFree.hpp:3858-3889):
  * begin launches the independent elements (they read owned u only); the caller
  * then fills u's ghost slots (amx_op_halo_import_dev); end launches the
- * dependent elements and the gather.  bc_operator as amx_op_matmult_dev when
+ * dependent elements and the gather (atomic scatter: begin zero-fills v and
+ * both add into it; no gather).  bc_operator as amx_op_matmult_dev when
  * apply_bc != 0; ghost slots of v then hold contributions for their owners
  * (amx_op_halo_export_dev / amx_op_halo_accumulate_dev), after which
  * amx_op_bc_fixup_dev applies the constrained rows. */
 int amx_op_matvec_begin_dev(amx_op* op, double* v, const double* u, int apply_bc, void* stream);
 int amx_op_matvec_end_dev(amx_op* op, double* v, const double* u, int apply_bc, void* stream);
+/* ghost_receive_begin's pack (include/aMatFree.hpp:1452-1454):
+ * send[i] = u[send_ids[i]], n_send entries (what the owner sends) */
+int amx_op_halo_pack_dev(amx_op* op, const double* u, double* send, void* stream);
 /* ghost_receive_end memcpy (include/aMatFree.hpp:1536-1539): halo[n_pre+n_post]
  * (pre then post) -> ghost slots of u.  Zeroes constrained ghosts when the
  * identity BC form is active, as the owner's MatMult_mf does. */
@@ -308,13 +360,18 @@ int amx_op_matmult_f32_dev(amx_op* op, int precision, float* v, const float* u, 
 
 /* aMatFree::MatGetDiagonal_mf (include/aMatFree.hpp:3944-3974):
  * mat_get_diagonal_ghosted (:997-1056) + apply_bc_diagonal (:4309-4337).
- * Host array, owned length.  _dev: device array, ghosted length. */
+ * Host array, owned length.  _dev: device array, ghosted length.  More than
+ * one rank: needs the halo transport (the owned rows then hold the assembled
+ * diagonal). */
 int amx_op_get_diagonal(amx_op* op, double* diag);
 int amx_op_get_diagonal_dev(amx_op* op, double* diag, void* stream);
 
 /* aMatFree::apply_bc_rhs (include/aMatFree.hpp:4536-4700): rhs (owned length,
  * host) -> rhs with constrained rows set (prescribed value, or penalty *
- * prescribed value) and, for BC_IMATRIX, free rows reduced by K_fc U_c. */
+ * prescribed value) and, for BC_IMATRIX, free rows reduced by K_fc U_c.
+ * More than one rank with BC_IMATRIX: K_fc U_c of the ghost rows reaches
+ * their owners through the halo transport (the reference's ADD_VALUES into
+ * a PETSc vector), which must be registered. */
 int amx_op_apply_bc_rhs(amx_op* op, double* rhs);
 
 /* aMatFree::get_timer (include/aMatFree.hpp:239-245): seconds accumulated

@@ -129,7 +129,8 @@ def test_atomic_scatter_and_determinism(torch_gpu, which):
     assert rel_l2(va, v1) <= 1e-13
 
 
-@pytest.mark.parametrize("prec,bound", [(PREC.FP32, 2e-6), (PREC.BF16, 2e-2)])
+# about twice the measured errors (FP32 8.7e-8, BF16 9.3e-4)
+@pytest.mark.parametrize("prec,bound", [(PREC.FP32, 2e-7), (PREC.BF16, 2e-3)])
 def test_low_precision_storage_error(torch_gpu, prec, bound):
     p = problems.elasticity_hex20(4)
     op = build(p, precision=prec)
@@ -323,8 +324,8 @@ def test_two_rank_partition_with_halo_on_one_gpu(torch_gpu):
 def test_elasticity_hex20_64_bitexact(torch_gpu):
     """Full size: 64^3 hex20 elasticity (262,144 elements,
     3,257,475 DoFs), K_e = E_e K_ref; the plain product is bit-exact against
-    the oracle, the penalty operator agrees through the diagonal identity,
-    and symmetry holds."""
+    the oracle, the penalty operator's free rows equal it bit for bit and its
+    constrained rows agree through the diagonal identity, and symmetry holds."""
     p = problems.elasticity_hex20(64)
     assert p.n_dofs == 3_257_475 and p.n_elem == 262_144
     op = build(p, bcop=BCOP.PENALTY_DIAG, penalty=1e8)
@@ -337,9 +338,12 @@ def test_elasticity_hex20_64_bitexact(torch_gpu):
     d = orc.get_diagonal(p.gmap, K, None, 0, p.n_dofs)
     del K
     c = p.constrained.astype(np.int64)
+    free = np.ones(p.n_dofs, dtype=bool)
+    free[c] = False
+    np.testing.assert_array_equal(vp[free], vo[free])   # free rows: the plain product, bit for bit
     ref = vo.copy()
     ref[c] += (1e8 - d[c]) * u[c]
-    assert rel_l2(vp, ref) <= 1e-13
+    assert rel_l2(vp[c], ref[c]) <= 1e-13
     w = mesh.uniform(77, 0, p.n_dofs, -1, 1)
     kw = run(torch_gpu, op.matvec_dev, w)
     assert abs(np.dot(w, v) - np.dot(u, kw)) <= 1e-12 * np.linalg.norm(w) * np.linalg.norm(v)

@@ -21,7 +21,8 @@ from conftest import rel_l2
 pytestmark = pytest.mark.gpu
 
 TOL_MODEL = 2e-6
-TOL_FP64 = {PREC.FP32: 1e-6, PREC.BF16: 1e-2}
+# about twice the largest measured errors against the FP64 operator
+TOL_FP64 = {PREC.FP32: 2e-7, PREC.BF16: 4e-3}
 
 
 def bf16_round(a: np.ndarray) -> np.ndarray:
@@ -73,6 +74,7 @@ def check(p, precision, U, V):
         ref_64 = orc.matvec(p.gmap, K64, np.ascontiguousarray(U[:, i].astype(np.float64)))
         worst_model = max(worst_model, rel_l2(V[:, i].astype(np.float64), ref_model))
         worst_64 = max(worst_64, rel_l2(V[:, i].astype(np.float64), ref_64))
+    print(f"{p.name} {PREC(precision).name}: vs rounded model {worst_model:.2e}, vs FP64 {worst_64:.2e}")
     assert worst_model <= TOL_MODEL, (p.name, worst_model)
     assert worst_64 <= TOL_FP64[precision], (p.name, worst_64)
     return worst_model, worst_64
