@@ -89,6 +89,27 @@ def build_dropin_example() -> pathlib.Path | None:
     return out
 
 
+MPI_HOME = pathlib.Path(os.environ.get("MPI_HOME", "/opt/conda"))
+
+
+def build_petsc_shell_example() -> pathlib.Path | None:
+    """tests/cpp/petsc_shell_example: the BUILD_WITH_PETSC form of the C++
+    drop-in against the test double tests/cpp/petsc_stub/petsc.h and MPICH
+    (skipped when MPICH is absent).  libmpi is linked by path and the system
+    library directory leads the run path, so that the HIP runtime finds the
+    system libstdc++ rather than the older one beside MPICH."""
+    src = ROOT / "tests" / "cpp" / "petsc_shell_example.cpp"
+    libmpi = MPI_HOME / "lib" / "libmpi.so.12"
+    if not src.exists() or not (MPI_HOME / "include" / "mpi.h").exists() or not libmpi.exists():
+        return None
+    out = src.with_suffix("")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-DBUILD_WITH_PETSC", f"-I{ROOT / 'include'}",
+          f"-I{ROOT / 'tests' / 'cpp' / 'petsc_stub'}", f"-I{MPI_HOME / 'include'}", str(src), f"-L{LIB.parent}",
+          "-lamat_amd", str(libmpi), f"-Wl,-rpath,{LIB.parent}:/usr/lib/x86_64-linux-gnu:{MPI_HOME / 'lib'}",
+          "-o", str(out)])
+    return out
+
+
 def build_oracle() -> None:
     if shutil.which("make") is None:
         raise RuntimeError("make is required to build the oracle")
@@ -101,6 +122,7 @@ def main() -> None:
     build_library()
     build_oracle()
     build_dropin_example()
+    build_petsc_shell_example()
     print(f"built {LIB}")
 
 

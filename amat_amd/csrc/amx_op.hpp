@@ -177,6 +177,19 @@ void apply(Op& op, KType kt, bool vt_double, int bc, const void* u, void* v, hip
 // identity rows, the penalty for penalty rows); scratch: n_total doubles.
 void build_dinv(Op& op, double* dinv, double* scratch, hipStream_t s);
 
+// Workspace of one FP64 CG solve: five n_total vectors, the reduction
+// partials (kPartialsBytes, counters at zero), the device scalars and an
+// optional pinned host copy of them (NULL: read through a local)
+struct CgWork {
+    double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *dinv = nullptr, *partials = nullptr;
+    CgScalars* sc = nullptr;
+    CgScalars* host = nullptr;
+};
+// FP64 Jacobi-PCG with true-residual replacements on workspace w (the body
+// of amx_cg_solve_dev); allocates nothing
+void fp64_cg(Op& op, const double* b, double* x, double rtol, int max_it, const CgWork& w, amx_solve_stats* stats,
+             hipStream_t s);
+
 }  // namespace amx
 
 struct amx_op {

@@ -57,8 +57,10 @@ void ensure_ws(Op& op)
     }
 }
 
-void set_scalars(Op& op, CgScalars* dsc, double rtol, int max_it, hipStream_t s, double delta = 0.0,
-                 int max_upd = 0)
+// host copies of the device scalars go through `host` (the handle's pinned
+// copy), or through a local when it is NULL
+void set_scalars_via(CgScalars* host, CgScalars* dsc, double rtol, int max_it, hipStream_t s, double delta = 0.0,
+                     int max_upd = 0)
 {
     CgScalars h{};
     h.tol2 = rtol * rtol;
@@ -66,29 +68,38 @@ void set_scalars(Op& op, CgScalars* dsc, double rtol, int max_it, hipStream_t s,
     h.d2 = delta * delta;
     h.max_upd = (uint32_t)max_upd;
     h.counter = 0;
-    *op.host_sc = h;
-    AMX_HIP(hipMemcpyAsync(dsc, op.host_sc, sizeof(CgScalars), hipMemcpyHostToDevice, s));
+    CgScalars* src = host ? host : &h;
+    *src = h;
+    AMX_HIP(hipMemcpyAsync(dsc, src, sizeof(CgScalars), hipMemcpyHostToDevice, s));
     AMX_HIP(hipStreamSynchronize(s));
 }
 
-CgScalars read_scalars(Op& op, const CgScalars* dsc, hipStream_t s)
+CgScalars read_scalars_via(CgScalars* host, const CgScalars* dsc, hipStream_t s)
 {
-    AMX_HIP(hipMemcpyAsync(op.host_sc, dsc, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
+    CgScalars h{};
+    CgScalars* dst = host ? host : &h;
+    AMX_HIP(hipMemcpyAsync(dst, dsc, sizeof(CgScalars), hipMemcpyDeviceToHost, s));
     AMX_HIP(hipStreamSynchronize(s));
-    return *op.host_sc;
+    return *dst;
 }
+
+void set_scalars(Op& op, CgScalars* dsc, double rtol, int max_it, hipStream_t s, double delta = 0.0,
+                 int max_upd = 0)
+{
+    set_scalars_via(op.host_sc, dsc, rtol, max_it, s, delta, max_upd);
+}
+
+CgScalars read_scalars(Op& op, const CgScalars* dsc, hipStream_t s) { return read_scalars_via(op.host_sc, dsc, s); }
 
 void build_dinv_ws(Op& op, hipStream_t s) { amx::build_dinv(op, op.ws_dinv.as<double>(), op.ws_q.as<double>(), s); }
 
 // PCG on A x = b (x starts at 0), vectors of type f64 ? double : float, the
 // operator with storage kt and the handle's BC form.  Returns iterations.
 // PCG iterations from the current state until the device stops them.
-CgScalars cg_loop(Op& op, KType kt, bool f64, void* x, void* r, void* z, void* p, void* q, int max_it,
-                  CgScalars* dsc, hipStream_t s)
+CgScalars cg_loop_ws(Op& op, KType kt, bool f64, void* x, void* r, void* z, void* p, void* q, int max_it,
+                     CgScalars* dsc, double* partials, const double* dinv, CgScalars* host, hipStream_t s)
 {
     const uint32_t n = op.n_total;
-    double* partials = op.ws_partials.as<double>();
-    const double* dinv = op.ws_dinv.as<double>();
     // poll every `chunk` iterations: about 1/8 of the iterations of a small
     // solve, every other one of a large solve
     const uint32_t chunk = n > (1u << 21) ? 2 : (n > (1u << 17) ? 4 : 8);
@@ -101,9 +112,16 @@ CgScalars cg_loop(Op& op, KType kt, bool f64, void* x, void* r, void* z, void* p
             cg_p(f64, p, z, n, dsc, s);
         }
         launched += (int)chunk;
-        const CgScalars sc = read_scalars(op, dsc, s);
+        const CgScalars sc = read_scalars_via(host, dsc, s);
         if (sc.done || launched >= max_it) return sc;
     }
+}
+
+CgScalars cg_loop(Op& op, KType kt, bool f64, void* x, void* r, void* z, void* p, void* q, int max_it,
+                  CgScalars* dsc, hipStream_t s)
+{
+    return cg_loop_ws(op, kt, f64, x, r, z, p, q, max_it, dsc, op.ws_partials.as<double>(),
+                      op.ws_dinv.as<double>(), op.host_sc, s);
 }
 
 CgScalars pcg(Op& op, KType kt, bool f64, const void* b, void* x, void* r, void* z, void* p, void* q, double rtol,
@@ -177,6 +195,43 @@ CgScalars ru_pcg(Op& op, KType kt, bool f64v, const double* b, double* x, void* 
 
 }  // namespace
 
+// FP64 Jacobi-PCG with true-residual replacements (amx_cg_solve_dev) on the
+// workspace w; returns the final scalars
+void fp64_cg(Op& op, const double* b, double* x, double rtol, int max_it, const CgWork& w, amx_solve_stats* stats,
+             hipStream_t s)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint32_t n = op.n_total;
+    build_dinv(op, w.dinv, w.q, s);
+    CgScalars* dsc = w.sc;
+    set_scalars_via(w.host, dsc, rtol, max_it, s, 0.0, kCgReplacements);
+    cg_init(true, b, w.dinv, x, w.r, w.z, w.p, n, w.partials, dsc, s);
+    CgScalars sc = cg_loop_ws(op, KType::F64, true, x, w.r, w.z, w.p, w.q, max_it, dsc, w.partials, w.dinv, w.host, s);
+    // the recurrence stopped: its residual is replaced by the true b - A x
+    // (FP64); the solve ends there when that meets rtol (or a limit is
+    // reached), else CG continues the direction from the true residual
+    CgScalars fin;
+    for (;;) {
+        apply(op, KType::F64, true, op.bc_operator, x, w.q, s);
+        ru_replace(true, false, b, w.q, w.r, w.z, w.p, w.dinv, n, w.partials, dsc, s);
+        cg_p(true, w.p, w.z, n, dsc, s);  // a no-op once final
+        fin = read_scalars_via(w.host, dsc, s);
+        if (fin.done || fin.breakdown) break;
+        sc = cg_loop_ws(op, KType::F64, true, x, w.r, w.z, w.p, w.q, max_it, dsc, w.partials, w.dinv, w.host, s);
+        if (sc.breakdown) break;
+    }
+    const double rel = fin.bb > 0 ? std::sqrt(fin.rr / fin.bb) : 0.0;
+    if (stats) {
+        stats->iterations = (int32_t)fin.iter;
+        stats->outer_iterations = (int32_t)fin.n_upd - 1;
+        stats->converged = rel <= rtol * 1.0000001 || fin.bb == 0.0;
+        stats->precision = AMX_PREC_FP64;
+        stats->rel_residual = rel;
+        stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    AMX_REQUIRE(!fin.breakdown, AMX_NOT_CONVERGED, "CG breakdown: p^T A p <= 0 (operator not SPD)");
+}
+
 }  // namespace amx
 
 extern "C" {
@@ -192,36 +247,17 @@ int amx_cg_solve_dev(amx_op* h, const double* b, double* x, double rtol, int max
         AMX_REQUIRE(op.size == 1, AMX_NOT_IMPLEMENTED, "device solver is one-rank");
         amx::DeviceGuard dg(op.device);
         hipStream_t s = op.pick(stream);
-        const auto t0 = std::chrono::steady_clock::now();
         amx::ensure_ws(op);
-        amx::build_dinv_ws(op, s);
-        amx::CgScalars* dsc = op.ws_sc.as<amx::CgScalars>();
-        amx::CgScalars sc = amx::pcg(op, KType::F64, true, b, x, op.ws_r.p, op.ws_z.p, op.ws_p.p, op.ws_q.p, rtol,
-                                     max_it, dsc, s, amx::kCgReplacements);
-        // the recurrence stopped: its residual is replaced by the true b - A x
-        // (FP64); the solve ends there when that meets rtol (or a limit is
-        // reached), else CG continues the direction from the true residual
-        amx::CgScalars fin;
-        for (;;) {
-            amx::apply(op, KType::F64, true, op.bc_operator, x, op.ws_q.p, s);
-            amx::ru_replace(true, false, b, op.ws_q.as<double>(), op.ws_r.p, op.ws_z.p, op.ws_p.p,
-                            op.ws_dinv.as<double>(), op.n_total, op.ws_partials.as<double>(), dsc, s);
-            amx::cg_p(true, op.ws_p.p, op.ws_z.p, op.n_total, dsc, s);  // a no-op once final
-            fin = amx::read_scalars(op, dsc, s);
-            if (fin.done || fin.breakdown) break;
-            sc = amx::cg_loop(op, KType::F64, true, x, op.ws_r.p, op.ws_z.p, op.ws_p.p, op.ws_q.p, max_it, dsc, s);
-            if (sc.breakdown) break;
-        }
-        const double rel = fin.bb > 0 ? std::sqrt(fin.rr / fin.bb) : 0.0;
-        if (stats) {
-            stats->iterations = (int32_t)fin.iter;
-            stats->outer_iterations = (int32_t)fin.n_upd - 1;
-            stats->converged = rel <= rtol * 1.0000001 || fin.bb == 0.0;
-            stats->precision = AMX_PREC_FP64;
-            stats->rel_residual = rel;
-            stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        }
-        AMX_REQUIRE(!fin.breakdown, AMX_NOT_CONVERGED, "CG breakdown: p^T A p <= 0 (operator not SPD)");
+        amx::CgWork w;
+        w.r = op.ws_r.as<double>();
+        w.z = op.ws_z.as<double>();
+        w.p = op.ws_p.as<double>();
+        w.q = op.ws_q.as<double>();
+        w.dinv = op.ws_dinv.as<double>();
+        w.partials = op.ws_partials.as<double>();
+        w.sc = op.ws_sc.as<amx::CgScalars>();
+        w.host = op.host_sc;
+        amx::fp64_cg(op, b, x, rtol, max_it, w, stats, s);
     });
 }
 

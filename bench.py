@@ -49,37 +49,62 @@ def cpu_baseline(p, n_sample_layers: int, seconds: float, threads: int) -> dict:
     K = p.element_matrices(0, ns)
     u = p.random_u()
 
-    def timed(layout):
+    def timed(layout, library=None):
         t0 = time.perf_counter()
-        orc.matvec(gm, K, u, n_total=p.n_dofs, nthreads=threads, layout=layout)  # warm-up and estimate
+        orc.matvec(gm, K, u, n_total=p.n_dofs, nthreads=threads, layout=layout, library=library)  # warm-up
         t1 = time.perf_counter() - t0
         reps = max(1, int(seconds / max(t1, 1e-6)))
         t0 = time.perf_counter()
         for _ in range(reps):
-            orc.matvec(gm, K, u, n_total=p.n_dofs, nthreads=threads, layout=layout)
+            orc.matvec(gm, K, u, n_total=p.n_dofs, nthreads=threads, layout=layout, library=library)
         return (time.perf_counter() - t0) / reps, reps
 
     dt, reps = timed(0)
     # the VECTORIZED_OPENMP variant (column AXPY with omp simd): K_e is symmetric, so
     # its column-major storage is the same array
     dt1, reps1 = timed(1)
-    t_full = dt * E / ns
-    model = ""
+    # the same loop built for this host with -O3 -march=native, the flags the
+    # reference's AVX builds add (CMakeLists.txt:93-104): the "OpenMP/AVX" number
+    import tempfile
+    with tempfile.TemporaryDirectory() as tmp:
+        nat = orc.native_library(tmp)
+        dt2, reps2 = timed(1, nat)
+    rates = {"default": p.n_dofs / (dt * E / ns) / 1e9, "vectorized_openmp": p.n_dofs / (dt1 * E / ns) / 1e9,
+             "vectorized_native": p.n_dofs / (dt2 * E / ns) / 1e9}
+    best = max(rates, key=rates.get)
+    t_best = {"default": dt, "vectorized_openmp": dt1, "vectorized_native": dt2}[best]
+    return {"value": rates[best], "unit": "GDoF/s", "cores": threads, "kind": "port",
+            "sample": f"{ns} of {E} elements ({ns // per_layer} element layers), about {seconds:.0f} s of matvecs "
+                      f"per variant, scaled by element count; oracle matvec_ghosted_OMP, {threads} OpenMP threads; "
+                      f"value = the fastest variant ({best}, {t_best * 1e3:.1f} ms per matvec)",
+            "cpu_model": cpu_model(),
+            "variants_GDoF_s": rates,
+            "variants": {"default": f"row-major scalar GEMV + omp atomic, -O2 (the reference's default build), "
+                                    f"{reps} matvecs, {dt * 1e3:.1f} ms each",
+                         "vectorized_openmp": f"column AXPY + omp simd (VECTORIZED_OPENMP), -O2, {reps1} matvecs, "
+                                              f"{dt1 * 1e3:.1f} ms each",
+                         "vectorized_native": f"column AXPY + omp simd, -O3 -march=native (the AVX builds' flags), "
+                                              f"{reps2} matvecs, {dt2 * 1e3:.1f} ms each"}}
+
+
+def cpu_model() -> str:
+    """The host CPU: model name, family/model/stepping and its widest SIMD."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                model = line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = (x.strip() for x in line.split(":", 1))
+                if k in ("model name", "cpu family", "model", "stepping", "flags") and k not in info:
+                    info[k] = v
+            if not line.strip() and info:
                 break
     except OSError:
-        pass
-    return {"value": p.n_dofs / t_full / 1e9, "unit": "GDoF/s", "cores": threads, "kind": "port",
-            "sample": f"{ns} of {E} elements ({ns // per_layer} element layers), {reps} matvecs "
-                      f"({dt * reps:.1f} s), {dt * 1e3:.1f} ms each, scaled by element count; oracle "
-                      f"matvec_ghosted_OMP (row-major scalar GEMV + omp atomic), {threads} OpenMP threads",
-            "cpu_model": model,
-            "vectorized_openmp_value": p.n_dofs / (dt1 * E / ns) / 1e9,
-            "vectorized_openmp_sample": f"{reps1} matvecs, {dt1 * 1e3:.1f} ms each (column AXPY + omp simd, "
-                                        f"the oracle built -O2 without -march)"}
+        return ""
+    flags = set(info.get("flags", "").split())
+    simd = next((f for f in ("avx512f", "avx2", "avx", "sse4_2") if f in flags), "sse2")
+    return (f"{info.get('model name', '?')} (family {info.get('cpu family', '?')} model {info.get('model', '?')} "
+            f"stepping {info.get('stepping', '?')}, {simd}{', avx512_bf16' if 'avx512_bf16' in flags else ''}, "
+            f"{os.cpu_count()} logical CPUs visible)")
 
 
 def load_traffic(name: str):
@@ -133,11 +158,12 @@ def host_boundary(op, u_dev, reps: int, torch) -> dict:
     return out
 
 
-def precision_mode(args) -> dict:
+def precision_mode(args, lite: bool = False) -> dict:
     """Precision: jittered tet10 Poisson (n^3 hexes x 6 tets), b = M f, identity
     BCs; FP64 CG vs FP64 iterative refinement with FP32 / BF16 element
     matrices (restarted per outer step, and with reliable updates), all to
-    1e-10 relative residual, error against the FP64 solution."""
+    1e-10 relative residual, error against the FP64 solution.  lite: the
+    reliable-update forms only, no method-2 operator."""
     import numpy as np
     import torch
     from amat_amd import BCOP, PREC, Maps, aMatFree, problems
@@ -179,7 +205,9 @@ def precision_mode(args) -> dict:
     from amat_amd.amat import IR_RELIABLE
     # restarted refinement (inner tolerance --inner-rtol) and reliable updates
     # (replacement threshold --ru-delta)
-    for prec, reliable in ((PREC.FP32, False), (PREC.BF16, False), (PREC.FP32, True), (PREC.BF16, True)):
+    forms = ((PREC.FP32, True), (PREC.BF16, True)) if lite else \
+        ((PREC.FP32, False), (PREC.BF16, False), (PREC.FP32, True), (PREC.BF16, True))
+    for prec, reliable in forms:
         x = torch.zeros_like(db)
         torch.cuda.synchronize()
         t = time.perf_counter()
@@ -196,6 +224,8 @@ def precision_mode(args) -> dict:
         log(f"[precision] {'RU' if reliable else 'IR'}-CG {prec.name}: {'replacements' if reliable else 'outer'} "
             f"{si.outer_iterations} inner {si.iterations}, {ti:.3f} s, relres {si.rel_residual:.2e}, "
             f"err vs FP64 {err:.2e}, speedup {t64 / ti:.2f}x")
+    if lite:
+        return res
     # method 2 (FREE): the same operator recomputed in-kernel from the coordinates,
     # FP32 inner solves with FP64 reliable updates (nothing stored per precision)
     del op
@@ -394,6 +424,43 @@ def batch_mode(args) -> dict:
     return res
 
 
+def extras(args) -> dict:
+    """The precision and batching workloads measured in the default run, in
+    brief: jittered tet10 Poisson 96^3 x 6 (matvec per storage, FP64 CG
+    against FP32 / BF16 storage with FP64 reliable updates, to 1e-10), and the
+    multi-RHS products of hex20 48^3 + hex8 80x40x20 (best autotuned launch
+    per storage and nrhs over both scatter forms)."""
+    import copy
+    import torch
+    out = {}
+    a = copy.copy(args)
+    a.n, a.steps, a.warmup = 96, min(args.steps, 20), 3
+    t0 = time.time()
+    pr = precision_mode(a, lite=True)
+    torch.cuda.empty_cache()
+    keep = ("seconds", "iterations", "inner", "outer", "rel_residual", "converged", "rel_error_vs_fp64_solution",
+            "speedup_vs_fp64_cg")
+    out["precision_tet10_96"] = {
+        "elements": pr["elements"], "dofs": pr["dofs"],
+        "matvec": {k[7:]: {"ms": v["ms"], "hbm_frac": v["hbm_frac"]} for k, v in pr.items() if k.startswith("matvec_")},
+        **{k: {f: v[f] for f in keep if f in v} for k, v in pr.items() if k.startswith(("cg_", "ru_cg"))},
+        "seconds_in_bench": time.time() - t0}
+    t0 = time.time()
+    b = copy.copy(args)
+    b.steps, b.warmup = min(args.steps, 20), 3
+    br = batch_mode(b)
+    torch.cuda.empty_cache()
+    best = {}
+    for c in br["cases"]:
+        key = f"{c['precision']}_nrhs{c['nrhs']}"
+        ms = min(c["ms"], c["ms_graph"])
+        if key not in best or ms < best[key]["ms"]:
+            best[key] = {"ms": ms, "scatter": c["scatter"], "tile": c["tile"], "hbm_frac": c["hbm_frac"],
+                         "tflops": c["tflops"], "col0_rel_err_vs_fp64": c["col0_rel_err_vs_fp64"]}
+    out["batch_hex20_48_hex8_80x40x20"] = {"best": best, "seconds_in_bench": time.time() - t0}
+    return out
+
+
 def serve_mode(args) -> dict:
     """Serving: a stream of CG solve requests (60% hex8 Poisson n~U{16..96},
     30% hex20 elasticity n~U{8..40}, 10% tet10 Poisson n~U{12..48}, own RHS,
@@ -492,8 +559,10 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=64, help="elements per cube edge")
     ap.add_argument("--scatter", choices=["deterministic", "atomic"], default="deterministic")
     ap.add_argument("--cpu-layers", type=int, default=1 << 30, help="element layers in the CPU baseline sample (all)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU time of the baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU time of each baseline variant")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the precision (tet10 96^3) and batching (multi-RHS) workloads of the default run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed region, copy 1 GiB device-to-device 5 times (known bytes for the "
                          "FETCH_SIZE / WRITE_SIZE counter scale)")
@@ -586,6 +655,10 @@ def main() -> int:
                      "alg_bytes_per_launch": elem_bytes, "ms_per_launch": ms_elem, "gather_ms": ms_gather},
         "host_arrays_pcie_inclusive": host,
     }
+    if not args.no_extra:
+        del op, u, v
+        torch.cuda.empty_cache()
+        out["extra"] = extras(args)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         log(f"[bench] CPU baseline on {threads} threads ...")

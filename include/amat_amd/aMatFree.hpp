@@ -16,17 +16,29 @@
  *   A.finalize();
  *   A.matvec(v, u, true);
  *
- * Differences: Maps takes (rank, size) instead of an MPI communicator (the
- * halo transport belongs to the caller, see amx_maps_get_ghost_requests);
- * failures throw par::amx_error carrying the par::Error code instead of
- * exiting; method 1 reports NOT_IMPLEMENTED.  Method 2 (FREE) runs on the GPU
+ * Communicator: with MPI available (BUILD_WITH_PETSC, whose petsc.h brings
+ * mpi.h, or AMAT_AMD_WITH_MPI) Maps also takes the reference's MPI_Comm;
+ * set_map then does buildScatterMap's collectives (MPI_Allgather of the owned
+ * counts, MPI_Alltoall/Alltoallv of the ghost requests, include/maps.hpp:547,
+ * 704, 738) and aMatFree registers an MPI halo transport (the Isend/Irecv
+ * pairs of include/aMatFree.hpp:1426-1679), so matvec, MatMult_mf,
+ * get_diagonal and apply_bc exchange ghosts as the reference does.  Without
+ * MPI, Maps takes (rank, size) and a rank's share exchanges through a
+ * transport the caller registers (amx_op_set_halo_transport).
+ * Differences: failures throw par::amx_error carrying the par::Error code
+ * instead of exiting; method 1 reports NOT_IMPLEMENTED.  Method 2 (FREE) runs on the GPU
  * in two forms: set_element_function (built-in element kinds recomputed
  * in-kernel from coordinates at every matvec) or the reference's
  * set_element_matrix_function callback, which is host code: it is evaluated
  * once per element at finalize and its matrices are stored (the operator then
  * runs as method 3, with the callback's values).  The PETSc shell hook
- * (get_matrix) and apply_bc(Vec) are compiled only with BUILD_WITH_PETSC;
- * dump_mat writes "row col value" lines instead of PETSc's MatView.
+ * (get_matrix with the reference's aMatCTX context and its three callbacks,
+ * MatMult_mf, MatGetDiagonal_mf, MatGetDiagonalBlock_mf) and apply_bc(Vec)
+ * are compiled only with BUILD_WITH_PETSC; par::solve
+ * (examples/include/solve.hpp:31-83) runs against them unchanged,
+ * except that block Jacobi (MATOP_GET_DIAGONAL_BLOCK) reports
+ * PETSC_ERR_SUP.  dump_mat writes "row col value" lines instead of PETSc's
+ * MatView.
  */
 #ifndef AMAT_AMD_AMATFREE_HPP
 #define AMAT_AMD_AMATFREE_HPP
@@ -44,6 +56,10 @@
 
 #ifdef BUILD_WITH_PETSC
 #include <petsc.h>
+#define AMAT_AMD_HAVE_MPI 1
+#elif defined(AMAT_AMD_WITH_MPI)
+#include <mpi.h>
+#define AMAT_AMD_HAVE_MPI 1
 #endif
 
 namespace par {
@@ -89,6 +105,21 @@ template <typename DT, typename GI, typename LI> class Maps {
         detail::check_types<DT, GI, LI>();
         detail::check(amx_maps_create(rank, size, &m_h), "Maps");
     }
+#ifdef AMAT_AMD_HAVE_MPI
+    /* Maps::Maps(MPI_Comm) (include/maps.hpp:328-348) */
+    explicit Maps(MPI_Comm comm) : m_comm(comm), m_has_comm(true)
+    {
+        detail::check_types<DT, GI, LI>();
+        int rank = 0, size = 1;
+        MPI_Comm_rank(comm, &rank);
+        MPI_Comm_size(comm, &size);
+        detail::check(amx_maps_create((unsigned)rank, (unsigned)size, &m_h), "Maps");
+    }
+    /* Maps::get_comm (include/maps.hpp:112-115); maps made from (rank, size)
+     * answer MPI_COMM_SELF on one rank, MPI_COMM_WORLD otherwise */
+    MPI_Comm get_comm() const { return m_has_comm ? m_comm : (info().size == 1 ? MPI_COMM_SELF : MPI_COMM_WORLD); }
+    bool has_comm() const { return m_has_comm; }
+#endif
     ~Maps() { amx_maps_destroy(m_h); }
     Maps(const Maps&) = delete;
     Maps& operator=(const Maps&) = delete;
@@ -101,12 +132,25 @@ template <typename DT, typename GI, typename LI> class Maps {
         std::vector<uint32_t> flat;
         for (LI e = 0; e < n_elements_on_rank; e++)
             flat.insert(flat.end(), element_to_rank_map[e], element_to_rank_map[e] + dofs_per_element[e]);
+#ifdef AMAT_AMD_HAVE_MPI
+        // buildScatterMap's MPI_Allgather of the owned counts (include/maps.hpp:547-548)
+        std::vector<LI> counts;
+        if (m_has_comm && rank_dof_counts == nullptr && info().size > 1) {
+            unsigned own = (unsigned)(owned_global_dof_range_end - owned_global_dof_range_begin + 1);
+            counts.resize(info().size);
+            MPI_Allgather(&own, 1, MPI_UNSIGNED, counts.data(), 1, MPI_UNSIGNED, m_comm);
+            rank_dof_counts = counts.data();
+        }
+#endif
         detail::check(amx_maps_set_map(m_h, n_elements_on_rank, flat.data(), dofs_per_element, n_all_dofs_on_rank,
                                        reinterpret_cast<const uint64_t*>(rank_to_global_map),
                                        owned_global_dof_range_begin, owned_global_dof_range_end, n_global_dofs,
                                        rank_dof_counts),
                       "Maps::set_map");
         m_dpe.assign(dofs_per_element, dofs_per_element + n_elements_on_rank);
+#ifdef AMAT_AMD_HAVE_MPI
+        if (m_has_comm && info().size > 1) exchange_ghost_requests();
+#endif
         return Error::SUCCESS;
     }
 
@@ -139,8 +183,51 @@ template <typename DT, typename GI, typename LI> class Maps {
         return out;
     }
     const amx_maps* handle() const { return m_h; }
+    /* send/recv lists (include/maps.hpp:741-785): per-rank counts and offsets */
+    void get_exchange_lists(std::vector<LI>& send_counts, std::vector<LI>& send_offsets, std::vector<LI>& recv_counts,
+                            std::vector<LI>& recv_offsets) const
+    {
+        const amx_maps_info i = info();
+        send_counts.assign(i.size, 0);
+        send_offsets.assign(i.size, 0);
+        recv_counts.assign(i.size, 0);
+        recv_offsets.assign(i.size, 0);
+        std::vector<LI> ids(std::max<uint32_t>(i.n_send, 1));
+        detail::check(amx_maps_get_send_lists(m_h, send_counts.data(), send_offsets.data(), ids.data()),
+                      "Maps::get_SendDofCounts");
+        detail::check(amx_maps_get_recv_lists(m_h, recv_counts.data(), recv_offsets.data()), "Maps::get_RecvDofCounts");
+    }
 
   private:
+#ifdef AMAT_AMD_HAVE_MPI
+    // buildScatterMap's MPI_Alltoall / MPI_Alltoallv of the ghost global ids
+    // (include/maps.hpp:700-739): each rank learns which of its DoFs its peers hold
+    void exchange_ghost_requests()
+    {
+        const amx_maps_info i = info();
+        const int P = (int)i.size;
+        std::vector<uint32_t> sc(P), rc(P);
+        std::vector<uint64_t> gids(std::max<uint32_t>(i.n_pre_ghost + i.n_post_ghost, 1));
+        detail::check(amx_maps_get_ghost_requests(m_h, sc.data(), gids.data()), "Maps::set_map");
+        MPI_Alltoall(sc.data(), 1, MPI_UINT32_T, rc.data(), 1, MPI_UINT32_T, m_comm);
+        std::vector<int> scnt(P), sdsp(P), rcnt(P), rdsp(P);
+        int st = 0, rt = 0;
+        for (int r = 0; r < P; r++) {
+            scnt[r] = (int)sc[r];
+            sdsp[r] = st;
+            st += scnt[r];
+            rcnt[r] = (int)rc[r];
+            rdsp[r] = rt;
+            rt += rcnt[r];
+        }
+        std::vector<uint64_t> peer(std::max(rt, 1));
+        MPI_Alltoallv(gids.data(), scnt.data(), sdsp.data(), MPI_UINT64_T, peer.data(), rcnt.data(), rdsp.data(),
+                      MPI_UINT64_T, m_comm);
+        detail::check(amx_maps_set_peer_requests(m_h, rc.data(), peer.data()), "Maps::set_map");
+    }
+    MPI_Comm m_comm = MPI_COMM_SELF;
+    bool m_has_comm = false;
+#endif
     amx_maps* m_h = nullptr;
     std::vector<LI> m_dpe;
 };
@@ -181,11 +268,43 @@ template <typename Derived, typename DT, typename GI, typename LI> class aMat {
     {
         return static_cast<Derived*>(this)->get_timer(scatterTime, gatherTime, mvTime, mvTotalTime);
     }
+#ifdef AMAT_AMD_HAVE_MPI
+    /* aMat::get_comm (include/aMat.hpp:119-121) */
+    MPI_Comm get_comm() { return static_cast<Derived*>(this)->get_comm(); }
+#endif
 #ifdef BUILD_WITH_PETSC
     Mat& get_matrix() { return static_cast<Derived*>(this)->get_matrix(); }
     Error apply_bc(Vec rhs) { return static_cast<Derived*>(this)->apply_bc(rhs); }
 #endif
 };
+
+#ifdef BUILD_WITH_PETSC
+template <typename DT, typename GI, typename LI> class aMatFree;
+/* context of the shell matrix (include/aMatFree.hpp:357-360): allocated by
+ * get_matrix, deleted by its user (par::solve, examples/include/solve.hpp:71-80) */
+template <typename DT, typename GI, typename LI> struct aMatCTX {
+    aMatFree<DT, GI, LI>* aMatPtr;
+};
+/* shell callbacks (include/aMatFree.hpp:363-401) */
+template <typename DT, typename GI, typename LI> PetscErrorCode aMat_matvec(Mat A, Vec u, Vec v)
+{
+    aMatCTX<DT, GI, LI>* ctx = nullptr;
+    MatShellGetContext(A, &ctx);
+    return ctx->aMatPtr->MatMult_mf(A, u, v);
+}
+template <typename DT, typename GI, typename LI> PetscErrorCode aMat_matgetdiagonal(Mat A, Vec d)
+{
+    aMatCTX<DT, GI, LI>* ctx = nullptr;
+    MatShellGetContext(A, &ctx);
+    return ctx->aMatPtr->MatGetDiagonal_mf(A, d);
+}
+template <typename DT, typename GI, typename LI> PetscErrorCode aMat_matgetdiagonalblock(Mat A, Mat* a)
+{
+    aMatCTX<DT, GI, LI>* ctx = nullptr;
+    MatShellGetContext(A, &ctx);
+    return ctx->aMatPtr->MatGetDiagonalBlock_mf(A, a);
+}
+#endif
 
 template <typename DT, typename GI, typename LI> class aMatFree : public aMat<aMatFree<DT, GI, LI>, DT, GI, LI> {
   public:
@@ -194,8 +313,21 @@ template <typename DT, typename GI, typename LI> class aMatFree : public aMat<aM
         : m_maps(mesh_maps)
     {
         detail::check(amx_op_create(mesh_maps.handle(), (int)bcType, device, &m_h), "aMatFree");
+#ifdef AMAT_AMD_HAVE_MPI
+        if (mesh_maps.has_comm() && mesh_maps.info().size > 1) {
+            m_comm = mesh_maps.get_comm();
+            mesh_maps.get_exchange_lists(m_sc, m_so, m_rc, m_ro);
+            detail::check(amx_op_set_halo_transport(m_h, &aMatFree::mpi_halo, this, AMX_HALO_HOST), "aMatFree");
+        }
+#endif
     }
-    ~aMatFree() { amx_op_destroy(m_h); }
+    ~aMatFree()
+    {
+#ifdef BUILD_WITH_PETSC
+        if (m_pMat_made) MatDestroy(&m_pMat);  // as ~aMatFree, include/aMatFree.hpp:523
+#endif
+        amx_op_destroy(m_h);
+    }
     aMatFree(const aMatFree&) = delete;
     aMatFree& operator=(const aMatFree&) = delete;
 
@@ -378,8 +510,12 @@ template <typename DT, typename GI, typename LI> class aMatFree : public aMat<aM
     const Maps<DT, GI, LI>& get_maps() { return m_maps; }
     amx_op* handle() { return m_h; }
 
+#ifdef AMAT_AMD_HAVE_MPI
+    /* aMat::get_comm (include/aMat.hpp:119-121): the communicator of the maps */
+    MPI_Comm get_comm() { return m_maps.get_comm(); }
+#endif
 #ifdef BUILD_WITH_PETSC
-    /* aMatFree::apply_bc(Vec) (include/aMatFree.hpp:957-971), one rank */
+    /* aMatFree::apply_bc(Vec) (include/aMatFree.hpp:957-971) */
     Error apply_bc(Vec rhs)
     {
         PetscScalar* r;
@@ -389,42 +525,86 @@ template <typename DT, typename GI, typename LI> class aMatFree : public aMat<aM
         detail::check(rc, "apply_bc");
         return Error::SUCCESS;
     }
-    /* aMatFree::get_matrix (include/aMatFree.hpp:535-570): a MATSHELL whose
-     * MATOP_MULT / MATOP_GET_DIAGONAL run on the GPU (one rank). */
+    /* aMatFree::get_matrix (include/aMatFree.hpp:535-570): a MATSHELL over the
+     * owned rows whose context is a heap aMatCTX pointing back here (its user
+     * deletes it, as par::solve does), with the reference's three operations */
     Mat& get_matrix()
     {
-        MatCreateShell(PETSC_COMM_SELF, m_maps.get_NumDofs(), m_maps.get_NumDofs(), PETSC_DETERMINE, PETSC_DETERMINE,
-                       this, &m_pMat);
-        MatShellSetOperation(m_pMat, MATOP_MULT, (void (*)(void))shell_mult);
-        MatShellSetOperation(m_pMat, MATOP_GET_DIAGONAL, (void (*)(void))shell_diag);
+        const LI n = m_maps.get_NumDofs();
+        aMatCTX<DT, GI, LI>* ctx = new aMatCTX<DT, GI, LI>();
+        ctx->aMatPtr = this;
+        if (m_pMat_made) MatDestroy(&m_pMat);
+        MatCreateShell(get_comm(), n, n, PETSC_DETERMINE, PETSC_DETERMINE, ctx, &m_pMat);
+        m_pMat_made = true;
+        MatShellSetOperation(m_pMat, MATOP_MULT, (void (*)(void))aMat_matvec<DT, GI, LI>);
+        MatShellSetOperation(m_pMat, MATOP_GET_DIAGONAL, (void (*)(void))aMat_matgetdiagonal<DT, GI, LI>);
+        MatShellSetOperation(m_pMat, MATOP_GET_DIAGONAL_BLOCK, (void (*)(void))aMat_matgetdiagonalblock<DT, GI, LI>);
         return m_pMat;
     }
-
-  private:
-    static PetscErrorCode shell_mult(Mat A, Vec u, Vec v)
+    /* aMatFree::MatMult_mf (include/aMatFree.hpp:3893-3941) on PETSc vectors */
+    PetscErrorCode MatMult_mf(Mat, Vec u, Vec v)
     {
-        aMatFree* self;
-        MatShellGetContext(A, &self);
         const PetscScalar* uu;
         PetscScalar* vv;
         VecGetArrayRead(u, &uu);
         VecGetArray(v, &vv);
-        const int rc = amx_op_matmult_mf(self->m_h, vv, uu);
+        const int rc = amx_op_matmult_mf(m_h, vv, uu);
         VecRestoreArrayRead(u, &uu);
         VecRestoreArray(v, &vv);
         return rc == AMX_SUCCESS ? 0 : PETSC_ERR_LIB;
     }
-    static PetscErrorCode shell_diag(Mat A, Vec d)
+    /* aMatFree::MatGetDiagonal_mf (include/aMatFree.hpp:3944-3974) */
+    PetscErrorCode MatGetDiagonal_mf(Mat, Vec d)
     {
-        aMatFree* self;
-        MatShellGetContext(A, &self);
         PetscScalar* dd;
         VecGetArray(d, &dd);
-        const int rc = amx_op_get_diagonal(self->m_h, dd);
+        const int rc = amx_op_get_diagonal(m_h, dd);
         VecRestoreArray(d, &dd);
         return rc == AMX_SUCCESS ? 0 : PETSC_ERR_LIB;
     }
-    Mat m_pMat;
+    /* aMatFree::MatGetDiagonalBlock_mf (include/aMatFree.hpp:3977-4031) feeds
+     * PETSc's block Jacobi, which this operator does not provide */
+    PetscErrorCode MatGetDiagonalBlock_mf(Mat, Mat*) { return PETSC_ERR_SUP; }
+
+  private:
+    Mat m_pMat = nullptr;
+    bool m_pMat_made = false;
+#endif
+#ifdef AMAT_AMD_HAVE_MPI
+    // The MPI exchange of ghost_receive_begin/end and ghost_send_begin/end
+    // (include/aMatFree.hpp:1426-1679) over host buffers: peer i's part of
+    // the owner-side buffer sits at send offset i, of the ghost-side buffer
+    // at recv offset i.
+    static int mpi_halo(void* ctx, int direction, const double* send, double* recv, void*)
+    {
+        aMatFree* self = static_cast<aMatFree*>(ctx);
+        const int P = (int)self->m_sc.size();
+        const bool to_ghost = direction == AMX_HALO_OWNER_TO_GHOST;
+        const int tag = to_ghost ? 7141 : 7142;
+        std::vector<MPI_Request> req;
+        req.reserve(2 * (size_t)P);
+        for (int i = 0; i < P; i++) {
+            const LI n_in = to_ghost ? self->m_rc[i] : self->m_sc[i];
+            const LI at_in = to_ghost ? self->m_ro[i] : self->m_so[i];
+            if (n_in) {
+                req.emplace_back();
+                MPI_Irecv(recv + at_in, (int)n_in, MPI_DOUBLE, i, tag, self->m_comm, &req.back());
+            }
+        }
+        for (int i = 0; i < P; i++) {
+            const LI n_out = to_ghost ? self->m_sc[i] : self->m_rc[i];
+            const LI at_out = to_ghost ? self->m_so[i] : self->m_ro[i];
+            if (n_out) {
+                req.emplace_back();
+                MPI_Isend(const_cast<double*>(send) + at_out, (int)n_out, MPI_DOUBLE, i, tag, self->m_comm,
+                          &req.back());
+            }
+        }
+        const int rc = req.empty() ? MPI_SUCCESS : MPI_Waitall((int)req.size(), req.data(), MPI_STATUSES_IGNORE);
+        return rc == MPI_SUCCESS ? 0 : 1;
+    }
+    MPI_Comm m_comm = MPI_COMM_SELF;
+    std::vector<LI> m_sc, m_so, m_rc, m_ro;
 #endif
 
   private:

@@ -28,6 +28,14 @@ def lib():
     return _lib
 
 
+def native_library(outdir) -> C.CDLL:
+    """The restatement built with -O3 -march=native for this host into
+    `outdir` (the CPU baseline's vectorized leg)."""
+    outdir = pathlib.Path(outdir)
+    subprocess.run(["make", "-s", "-C", str(_HERE), "native", f"NATIVE_OUT={outdir}"], check=True)
+    return C.CDLL(str(outdir / "liboracle_native.so"))
+
+
 def _p(a):
     # data_as keeps a reference to the array for as long as the pointer lives,
     # so temporaries stay alive during the call
@@ -109,9 +117,10 @@ def identify_independent(rank, size, rank_dof_counts, global_map_2d):
     return a[:na.value], b[:nb.value]
 
 
-def matvec(local_map, K, u, n_total=None, layout=0, nthreads=1, indep=None, dep=None):
+def matvec(local_map, K, u, n_total=None, layout=0, nthreads=1, indep=None, dep=None, library=None):
     """matvec_ghosted_OMP; K is (E, n, n) row-major (layout 0) or the same
-    matrices stored column-major per element (layout 1: pass K transposed)."""
+    matrices stored column-major per element (layout 1: pass K transposed).
+    library: another build of the restatement (native_library)."""
     lm = _u32(local_map)
     E, n = lm.shape
     Kf = _f64(K).reshape(-1)
@@ -120,7 +129,7 @@ def matvec(local_map, K, u, n_total=None, layout=0, nthreads=1, indep=None, dep=
     v = np.zeros(nt)
     ind = _u32(np.arange(E) if indep is None else indep)
     de = _u32(np.zeros(0) if dep is None else dep)
-    lib().orc_matvec_ghosted_omp(C.c_uint32(E), _p(_dpe(lm)), _p(lm), _p(Kf), C.c_int(layout), _p(ind),
+    (library or lib()).orc_matvec_ghosted_omp(C.c_uint32(E), _p(_dpe(lm)), _p(lm), _p(Kf), C.c_int(layout), _p(ind),
                                  C.c_uint32(len(ind)), _p(de) if len(de) else None, C.c_uint32(len(de)), _p(uu), _p(v),
                                  C.c_uint32(nt), C.c_int(nthreads))
     return v
