@@ -127,6 +127,7 @@ SIGNATURES = {
     "amx_op_stream": (P, [P]),
     "amx_op_synchronize": (I32, [P]),
     "amx_cg_solve_dev": (I32, [P, P, P, F64, I32, C.POINTER(SolveStats), P]),
+    "amx_cg_solve_noalloc_dev": (I32, [P, P, P, F64, I32, C.POINTER(SolveStats), P]),
     "amx_ir_cg_solve_dev": (I32, [P, I32, P, P, F64, F64, I32, I32, C.POINTER(SolveStats), P]),
     "amx_mrhs_create": (I32, [P, I32, I32, P]),
     "amx_mrhs_destroy": (I32, [P]),

@@ -103,6 +103,8 @@ class HALO(enum.IntEnum):
     DEVICE = 1   # device addresses and the stream (an RCCL caller)
 
 
+NOALLOC_MAX_DOFS = 1 << 20   # AMX_NOALLOC_MAX_DOFS: largest operator of cg_noalloc
+
 OWNER_TO_GHOST = 0   # ghost_receive_begin/end (include/aMatFree.hpp:1426-1549)
 GHOST_TO_OWNER = 1   # ghost_send_begin/end (:1552-1679)
 
@@ -586,6 +588,15 @@ class aMatFree:
     def cg(self, b, x, rtol=1e-10, max_it=10000, stream=None) -> _capi.SolveStats:
         st = _capi.SolveStats()
         call("amx_cg_solve_dev", self._h, self._vec(b, "b"), self._vec(x, "x"), float(rtol), int(max_it),
+             C.byref(st), _stream_ptr(stream))
+        return st
+
+    def cg_noalloc(self, b, x, rtol=1e-10, max_it=10000, stream=None) -> _capi.SolveStats:
+        """cg() with its workspace in the library's fixed device array
+        (amx_cg_solve_noalloc_dev): no device allocation; at most
+        NOALLOC_MAX_DOFS DoFs; calls overlapping on one GPU race."""
+        st = _capi.SolveStats()
+        call("amx_cg_solve_noalloc_dev", self._h, self._vec(b, "b"), self._vec(x, "x"), float(rtol), int(max_it),
              C.byref(st), _stream_ptr(stream))
         return st
 

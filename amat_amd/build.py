@@ -39,6 +39,7 @@ SOURCES = [
     "amx_solver_kernels.hip",
     "amx_mrhs.hip",
     "amx_free.hip",
+    "amx_noalloc.hip",
 ]
 
 COMMON = [

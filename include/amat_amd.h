@@ -414,6 +414,16 @@ typedef struct amx_solve_stats {
  * direction from it while it misses rtol (at most 4 replacements). */
 int amx_cg_solve_dev(amx_op* op, const double* b, double* x, double rtol, int max_it,
                      amx_solve_stats* stats, void* stream);
+/* No counterpart in the reference: amx_cg_solve_dev for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arithmetic and results; the CG
+ * vectors, reduction partials and scalars live in a fixed device array that
+ * each GPU holds until the process ends, so no call allocates.  Operators of
+ * more than AMX_NOALLOC_MAX_DOFS DoFs are rejected (AMX_INVALID_ARGUMENT).
+ * All calls on one GPU share that array: calls that overlap on one GPU race. */
+#define AMX_NOALLOC_MAX_DOFS 1048576u
+int amx_cg_solve_noalloc_dev(amx_op* op, const double* b, double* x, double rtol, int max_it,
+                             amx_solve_stats* stats, void* stream);
 /* FP64 iterative refinement: true residual, alpha/beta, inner products and the
  * penalty diagonal in FP64; inner Jacobi-PCG on the FP32 or BF16 operator to
  * inner_rtol.  precision = AMX_PREC_FP32 or AMX_PREC_BF16, optionally or-ed
