@@ -328,6 +328,10 @@ class aMatFree:
         self._h = h
         self.maps = mesh_maps
         self.device = device
+        # the operator copied these from the maps at creation; the argument
+        # checks of every call use them
+        inf = mesh_maps.info()
+        self._size_t = (inf.n_total, inf.n_owned, inf.n_pre_ghost + inf.n_post_ghost, inf.n_send)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -438,8 +442,8 @@ class aMatFree:
 
     # -- host-array forms ----------------------------------------------
     def _sizes(self):
-        inf = self.maps.info()
-        return inf.n_total, inf.n_owned, inf.n_pre_ghost + inf.n_post_ghost, inf.n_send
+        """(n_total, n_owned, ghosts, send entries) of the operator's share."""
+        return self._size_t
 
     def matvec(self, v: np.ndarray, u: np.ndarray, isGhosted: bool = False):
         """aMatFree::matvec (include/aMatFree.hpp:1681-1775): v = K u; length

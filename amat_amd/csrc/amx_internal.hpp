@@ -101,6 +101,7 @@ struct amx_maps {
     std::vector<uint64_t> dof_scan;          // m_ulvLocalDofScan
     uint32_t n_elem = 0;
     std::vector<uint32_t> dpe;               // dofs per element
+    uint32_t max_dpe = 0;                    // largest entry of dpe
     std::vector<uint64_t> eoff;              // prefix of dpe (n_elem + 1)
     std::vector<uint64_t> global_map;        // m_ulpMap flattened
     std::vector<uint32_t> local_map;         // m_uipLocalMap flattened

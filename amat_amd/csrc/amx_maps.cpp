@@ -165,9 +165,11 @@ int amx_maps_set_map(amx_maps* maps, uint32_t n_elements_on_rank, const uint32_t
         m.owned_end = owned_end;
         m.dpe.assign(dofs_per_element, dofs_per_element + n_elements_on_rank);
         m.eoff.assign(m.n_elem + 1, 0);
+        m.max_dpe = 0;
         for (uint32_t e = 0; e < m.n_elem; e++) {
             AMX_REQUIRE(m.dpe[e] > 0, AMX_INVALID_ARGUMENT, "element " + std::to_string(e) + " has no DoFs");
             m.eoff[e + 1] = m.eoff[e] + m.dpe[e];
+            m.max_dpe = std::max(m.max_dpe, m.dpe[e]);
         }
         if (m.size == 1) {
             m.rank_dof_counts.assign(1, m.n_owned);
@@ -300,7 +302,7 @@ int amx_maps_get_info(const amx_maps* maps, amx_maps_info* info)
         info->n_post_ghost = m.n_post;
         info->n_total = m.n_total;
         info->n_entries = m.eoff.empty() ? 0 : (uint32_t)m.eoff.back();
-        for (uint32_t d : m.dpe) info->max_dofs_per_elem = std::max(info->max_dofs_per_elem, d);
+        info->max_dofs_per_elem = m.max_dpe;
         info->n_independent = (uint32_t)m.independent.size();
         info->n_dependent = (uint32_t)m.dependent.size();
         info->n_owned_constraints = (uint32_t)m.owned_cdofs.size();
