@@ -378,7 +378,11 @@ def batch_mode(args) -> dict:
                               .reshape(p.n_dofs, nrhs)).cuda() for i, p in enumerate(probs)]
         V = [torch.empty_like(u) for u in U]
         with torch.cuda.stream(s):
-            tunes = mr.autotune(nrhs, U, V, reps=5, stream=s)
+            if args.batch_tiles:   # a fixed launch shape (profiling passes): no sweep
+                mr.set_tiles(*[int(x) for x in args.batch_tiles.split(",")])
+                tunes = {}
+            else:
+                tunes = mr.autotune(nrhs, U, V, reps=5, stream=s)
             for _ in range(args.warmup):
                 mr.apply_dev(nrhs, U, V, s)
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -389,15 +393,17 @@ def batch_mode(args) -> dict:
             e1.record(s)
             s.synchronize()
             ms = e0.elapsed_time(e1) / args.steps
-            mr.graph_capture(nrhs, U, V, s)
-            mr.graph_launch(s)
-            s.synchronize()
-            e0.record(s)
-            for _ in range(args.steps):
+            ms_graph = float("nan")
+            if not args.batch_tiles:
+                mr.graph_capture(nrhs, U, V, s)
                 mr.graph_launch(s)
-            e1.record(s)
-            s.synchronize()
-            ms_graph = e0.elapsed_time(e1) / args.steps
+                s.synchronize()
+                e0.record(s)
+                for _ in range(args.steps):
+                    mr.graph_launch(s)
+                e1.record(s)
+                s.synchronize()
+                ms_graph = e0.elapsed_time(e1) / args.steps
         # column 0 against the FP64 operator (plain element sum)
         errs = []
         for i, (p, op) in enumerate(zip(probs, ops)):
@@ -408,7 +414,7 @@ def batch_mode(args) -> dict:
         alg = sum(p.n_elem * (p.dofs_per_elem ** 2 * sk + 4 * p.dofs_per_elem) + p.n_dofs * 8 * nrhs
                   for p in probs)
         flops = sum(2 * p.n_elem * p.dofs_per_elem ** 2 * nrhs for p in probs)
-        best = min(ms, ms_graph)
+        best = min(ms, ms_graph) if ms_graph == ms_graph else ms
         case = {"precision": prec_name, "scatter": sc_name, "nrhs": nrhs, "tile": list(mr.tiles()), "ms": ms,
                 "ms_graph": ms_graph,
                 "GDoF_rhs_per_s": sum(p.n_dofs for p in probs) * nrhs / (best * 1e-3) / 1e9,
@@ -536,6 +542,17 @@ def serve_mode(args) -> dict:
     return res
 
 
+def calibrate_copy(torch) -> None:
+    """1 GiB copied device to device 5 times: known bytes for the FETCH_SIZE /
+    WRITE_SIZE counter scale of a profiling pass."""
+    src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--mode", choices=["matvec", "precision", "batch", "serve", "free"], default="matvec")
@@ -552,6 +569,8 @@ def main() -> int:
     ap.add_argument("--batch-nrhs", default="8,32,128")
     ap.add_argument("--batch-precisions", default="fp32,bf16")
     ap.add_argument("--batch-scatter", default="atomic,deterministic")
+    ap.add_argument("--batch-tiles", default="",
+                    help="rhs_tile,waves,split: a fixed launch shape instead of the autotuner (profiling passes)")
     ap.add_argument("--inner-rtol", type=float, default=1e-2)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
@@ -581,6 +600,8 @@ def main() -> int:
         return 0
     if args.mode == "batch":
         print(json.dumps(batch_mode(args)), flush=True)
+        if args.calibrate_copy:
+            calibrate_copy(torch)
         return 0
     if args.mode == "serve":
         print(json.dumps(serve_mode(args)), flush=True)
@@ -614,12 +635,7 @@ def main() -> int:
     ms_elem, ms_gather, ms_tot = op.time_kernels_dev(args.steps, v, u, s)
     log(f"[bench] kernel timers: element {ms_elem:.4f} ms, gather {ms_gather:.4f} ms")
     if args.calibrate_copy:
-        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        calibrate_copy(torch)
     # measured peak beside the nominal one: a 16-byte streaming read of 2 GiB
     from amat_amd.amat import measure_read_bandwidth
     read_bw = measure_read_bandwidth(torch.cuda.current_device(), 2 << 30, 10)
