@@ -261,7 +261,8 @@ class ServingLoop:
 
     Each turn: admit arrived requests (FIFO) while a slot is free and the
     operator + workspace fit the budget; run one step of the server; retire
-    finished jobs.  Latency = finish - arrival."""
+    finished jobs.  Latency = finish - arrival; service time = finish -
+    admission (the job on the GPU); queue wait = admission - arrival."""
 
     def __init__(self, server: Server, cache: OperatorCache, method: PREC, budget_bytes: int, n_iter: int = 8,
                  rtol: float = 1e-10, inner_rtol: float = 1e-2, keep_x: bool = False,
@@ -288,6 +289,8 @@ class ServingLoop:
         t0 = self.clock()
         peak_jobs = 0
         peak_bytes = 0
+        builds0, evict0, build_s0 = self.cache.builds, self.cache.evictions, self.cache.build_seconds
+        budget_waits = 0   # turns on which the head request waited for HBM (a slot was free)
         while incoming or pending or running:
             now = self.clock() - t0
             while incoming and (incoming[0].arrival <= now or not realtime):
@@ -302,6 +305,7 @@ class ServingLoop:
                 if not self.cache.evict_for(need_op + need_job, self.budget, workspace):
                     if not running:
                         raise RuntimeError(f"request {r.rid} ({r.kind} n={r.n}) cannot fit the HBM budget")
+                    budget_waits += 1
                     break
                 if c is None:
                     c = self.cache.build(key)
@@ -329,13 +333,26 @@ class ServingLoop:
                 time.sleep(max(0.0, min(incoming[0].arrival - (self.clock() - t0), 0.01)))
         makespan = self.clock() - t0
         lat = np.array([d.finished - d.arrival for d in done])
+        svc = np.array([d.finished - d.admitted for d in done])
+        wait = np.array([d.admitted - d.arrival for d in done])
+
+        def pct(a, q):
+            return float(np.percentile(a, q)) if len(a) else 0.0
         return {
             "done": done,
             "solves": len(done),
             "makespan_s": makespan,
             "solves_per_s": len(done) / makespan if makespan > 0 else 0.0,
-            "p50_s": float(np.percentile(lat, 50)) if len(lat) else 0.0,
-            "p99_s": float(np.percentile(lat, 99)) if len(lat) else 0.0,
+            "p50_s": pct(lat, 50),
+            "p99_s": pct(lat, 99),
+            "service_p50_s": pct(svc, 50),
+            "service_p99_s": pct(svc, 99),
+            "queue_wait_p50_s": pct(wait, 50),
+            "queue_wait_p99_s": pct(wait, 99),
+            "operator_builds": self.cache.builds - builds0,
+            "operator_build_s": self.cache.build_seconds - build_s0,
+            "evictions": self.cache.evictions - evict0,
+            "budget_waits": budget_waits,
             "all_converged": all(d.converged for d in done),
             "max_rel_residual": max((d.rel_residual for d in done), default=0.0),
             "peak_jobs": peak_jobs,

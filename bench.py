@@ -494,7 +494,7 @@ def serve_mode(args) -> dict:
     for free in sorted({n.endswith("free") for n in names}):
         cache = serve.OperatorCache(free=free)
         t0 = time.time()
-        for i, key in enumerate(keys):
+        for i, key in enumerate(keys if not args.serve_cold else []):
             cache.build(key)
             if i % 10 == 0:
                 log(f"[serve] built {i + 1}/{len(keys)} {'method-2 ' if free else ''}operators "
@@ -505,7 +505,7 @@ def serve_mode(args) -> dict:
     cache = caches.get(False, caches.get(True))
     res = {"metric": "served CG solves/s and latency at 1e-10 relative residual", "requests": args.serve_requests,
            "rates": rates, "budget_GiB": args.serve_budget_gb, "slots": args.serve_slots, "step_iters": args.serve_iters,
-           "inner_rtol": args.inner_rtol, "ru_delta": args.ru_delta,
+           "inner_rtol": args.inner_rtol, "ru_delta": args.ru_delta, "cold_start": bool(args.serve_cold),
            "operators": len(keys), "operator_GiB": cache.bytes / 2**30, "operator_build_s": cache.build_seconds,
            "runs": []}
     for rate in rates:
@@ -531,14 +531,22 @@ def serve_mode(args) -> dict:
             m.update({"rate": rate, "method": name, "by_kind": by_kind,
                       "iterations": int(sum(d.iterations for d in done))})
             res["runs"].append(m)
-            log(f"[serve] rate {rate:g} {name}: {m['solves_per_s']:.1f} solves/s, p50 {m['p50_s'] * 1e3:.0f} ms, "
-                f"p99 {m['p99_s'] * 1e3:.0f} ms, converged {m['all_converged']}, max relres {m['max_rel_residual']:.2e}")
+            log(f"[serve] rate {rate:g} {name}: {m['solves_per_s']:.1f} solves/s, latency p50 {m['p50_s'] * 1e3:.0f} ms "
+                f"p99 {m['p99_s'] * 1e3:.0f} ms, service p50 {m['service_p50_s'] * 1e3:.0f} ms p99 "
+                f"{m['service_p99_s'] * 1e3:.0f} ms, builds {m['operator_builds']} ({m['operator_build_s']:.1f}s), "
+                f"evictions {m['evictions']}, budget waits {m['budget_waits']}, converged {m['all_converged']}, "
+                f"max relres {m['max_rel_residual']:.2e}")
             del server
+    # same-form speedups (stored K: against FP64 CG on stored K; method 2: against FP64
+    # CG on the method-2 operators) and the cross-form one against stored FP64 CG
     for rate in rates:
-        base = [r for r in res["runs"] if r["rate"] == rate and r["method"] == "fp64"]
-        for r in res["runs"]:
-            if base and r["rate"] == rate:
-                r["speedup_vs_fp64"] = r["solves_per_s"] / base[0]["solves_per_s"]
+        runs = {r["method"]: r for r in res["runs"] if r["rate"] == rate}
+        for name, r in runs.items():
+            same = runs.get("fp64free" if name.endswith("free") else "fp64")
+            if same:
+                r["speedup_same_form"] = r["solves_per_s"] / same["solves_per_s"]
+            if "fp64" in runs:
+                r["speedup_vs_stored_fp64"] = r["solves_per_s"] / runs["fp64"]["solves_per_s"]
     return res
 
 
@@ -562,6 +570,8 @@ def main() -> int:
     ap.add_argument("--ru-delta", type=float, default=0.1,
                     help="reliable-update replacement threshold (methods fp32r, bf16r, --mode precision)")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-cold", action="store_true",
+                    help="start with no resident operator: builds, admission waits and evictions happen in the stream")
     ap.add_argument("--serve-slots", type=int, default=64)
     ap.add_argument("--serve-iters", type=int, default=8)
     ap.add_argument("--serve-seed", type=int, default=7)

@@ -106,6 +106,15 @@ def test_budget_slots_fifo_and_lru(monkeypatch):
     assert all(a <= b for a, b in zip(adm, adm[1:]))
     lat = [d.finished - d.arrival for d in m["done"]]
     assert m["p50_s"] == pytest.approx(float(np.percentile(lat, 50)))
+    # service time (admission -> finish) and queue wait (arrival -> admission) split the latency
+    svc = [d.finished - d.admitted for d in m["done"]]
+    wait = [d.admitted - d.arrival for d in m["done"]]
+    assert m["service_p99_s"] == pytest.approx(float(np.percentile(svc, 99)))
+    assert m["queue_wait_p50_s"] == pytest.approx(float(np.percentile(wait, 50)))
+    assert all(s > 0 for s in svc) and all(w >= 0 for w in wait)
+    # the budget bound inside the stream: builds, evictions and waits are counted by the run
+    assert m["operator_builds"] == cache.builds and m["evictions"] == cache.evictions > 0
+    assert m["budget_waits"] > 0
 
 
 def test_request_that_never_fits_raises(monkeypatch):
