@@ -58,7 +58,7 @@ class OpInfo(C.Structure):
         (n, C.c_int32) for n in ("matfree_type", "bc_meth", "bc_operator", "scatter_mode", "precision_mask")] + [
         ("trace", C.c_double), ("penalty", C.c_double)] + [
         (n, C.c_uint64) for n in ("device_bytes", "k_bytes_fp64", "k_bytes_fp32", "k_bytes_bf16", "alg_bytes_fp64",
-                                  "flops")]
+                                  "flops")] + [("packed_symmetric", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class SolveStats(C.Structure):
@@ -103,6 +103,7 @@ SIGNATURES = {
     "amx_measure_read_bandwidth": (I32, [I32, U64, I32, C.POINTER(F64)]),
     "amx_op_set_options": (I32, [P, I32, I32]),
     "amx_op_set_penalty": (I32, [P, F64]),
+    "amx_op_set_symmetric_storage": (I32, [P, I32]),
     "amx_op_set_bc_operator": (I32, [P, I32]),
     "amx_op_finalize": (I32, [P]),
     "amx_op_get_info": (I32, [P, C.POINTER(OpInfo)]),

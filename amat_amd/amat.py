@@ -382,6 +382,11 @@ class aMatFree:
     def set_options(self, precision=PREC.FP64, scatter=SCATTER.DETERMINISTIC):
         call("amx_op_set_options", self._h, int(precision), int(scatter))
 
+    def set_symmetric_storage(self, mode: int):
+        """1 (default): pack the upper triangles of 8-/10-DoF element matrices
+        when all are bitwise symmetric; 0: whole matrices (next finalize)."""
+        call("amx_op_set_symmetric_storage", self._h, int(mode))
+
     def set_penalty(self, p: float):
         call("amx_op_set_penalty", self._h, float(p))
 

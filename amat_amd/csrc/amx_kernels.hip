@@ -55,6 +55,19 @@ __host__ __device__ constexpr int pack_of(int n, int ksize)
                                               : 1;
 }
 
+// packed symmetric tiles: values of one element (upper triangle), index of
+// (r, c) in it, values per stored tile (G elements, padded to 16 bytes)
+__host__ __device__ constexpr uint32_t sym_ns(uint32_t n) { return n * (n + 1) / 2; }
+__host__ __device__ __forceinline__ uint32_t sym_idx(uint32_t n, uint32_t r, uint32_t c)
+{
+    const uint32_t a = r < c ? r : c, b = r < c ? c : r;
+    return a * n - (a * (a - 1)) / 2 + (b - a);
+}
+__host__ __device__ constexpr uint32_t sym_tile_values(uint32_t n, uint32_t ksize)
+{
+    return (((64 / n) * sym_ns(n) * ksize + 15) / 16 * 16) / ksize;
+}
+
 __device__ __forceinline__ float bf16_to_f32(uint32_t bits16) { return __uint_as_float(bits16 << 16); }
 
 __device__ __forceinline__ uint16_t f64_to_bf16(double x)
@@ -129,6 +142,19 @@ __device__ __forceinline__ double madd(double acc, double k, double u)
 }
 __device__ __forceinline__ float madd(float acc, float k, float u) { return __builtin_fmaf(k, u, acc); }
 
+// one stored value in the compute type
+__device__ __forceinline__ double kval(double x) { return x; }
+__device__ __forceinline__ float kval(float x) { return x; }
+__device__ __forceinline__ float kval(uint16_t x) { return bf16_to_f32(x); }
+
+// LDS written by this wave is visible to all its lanes
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // value of lane `src` (uniform, compile-time after unrolling) to all lanes
 __device__ __forceinline__ double readlane(double x, int src)
 {
@@ -159,10 +185,20 @@ template <typename VT> __device__ __forceinline__ void atomic_add(VT* p, VT x) {
 // consecutive elements together: what a wave of consecutive DoFs reads in the gather).
 //
 // ebe_tile_acc: lane (g, r) of tile t forms row r of K_e u_e (m: its map entry).
-template <int N, typename KT, typename VT, bool MASKED>
+//
+// SYM (packed symmetric tiles, G > 1): the wave copies its tile's upper
+// triangles (16-byte loads, all lanes) into its LDS region ksh, then lane
+// (g, r) reads K[r][c] = K[min][max] there, c ascending: the products and sums
+// of the full layout, in the same order.
+template <int N, typename KT> constexpr uint32_t sym_region_values()
+{
+    return (64 / N > 1) ? sym_tile_values(N, KSize<KT>::v) : 1u;
+}
+
+template <int N, typename KT, typename VT, bool MASKED, bool SYM = false>
 __device__ __forceinline__ VT ebe_tile_acc(const KT* __restrict__ K, const uint32_t* __restrict__ map,
                                            const VT* __restrict__ u, uint32_t t, const float* __restrict__ kdiag,
-                                           uint32_t& m)
+                                           uint32_t& m, KT* ksh = nullptr)
 {
     using AT = VT;
     constexpr int G = 64 / N;
@@ -174,12 +210,38 @@ __device__ __forceinline__ VT ebe_tile_acc(const KT* __restrict__ K, const uint3
     const uint32_t lc = lane < (uint32_t)GN ? lane : (uint32_t)GN - 1u;
     const uint32_t g = lc / N;
     const uint64_t row = (uint64_t)t * GN + lc;  // = slot * N + r
+    AT acc = AT(0);
+    if constexpr (SYM) {
+        static_assert(G > 1, "packed symmetric tiles hold several elements");
+        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+        constexpr uint32_t TE = sym_tile_values(N, KSize<KT>::v), NS = sym_ns(N);
+        constexpr uint32_t CH = TE * KSize<KT>::v / 16, NCH = (CH + 63) / 64;
+        const u32x4* src = reinterpret_cast<const u32x4*>(K + (uint64_t)t * TE);
+        u32x4 ch[NCH];
+#pragma unroll
+        for (uint32_t i = 0; i < NCH; i++)
+            if (lane + 64 * i < CH) ch[i] = __builtin_nontemporal_load(src + lane + 64 * i);
+        m = map[row];
+        VT uv = u[m & kIdxMask];
+        if (MASKED && (m & kFlagBit)) uv = VT(0);
+        const AT ue = uv;
+        u32x4* dst = reinterpret_cast<u32x4*>(ksh);
+#pragma unroll
+        for (uint32_t i = 0; i < NCH; i++)
+            if (lane + 64 * i < CH) dst[lane + 64 * i] = ch[i];
+        wave_lds_sync();
+        const uint32_t r = lc - g * N;
+        const KT* ke = ksh + g * NS;
+#pragma unroll
+        for (int c = 0; c < N; ++c) acc = madd(acc, (AT)kval(ke[sym_idx(N, r, c)]), bcast<G, N>(ue, g, c));
+        if constexpr (std::is_same<KT, uint16_t>::value) acc = madd(acc, (AT)kdiag[row], ue);
+        return acc;
+    }
     m = map[row];
     VT uv = u[m & kIdxMask];
     if (MASKED && (m & kFlagBit)) uv = VT(0);
     const AT ue = uv;
     const KT* kp = K + ((uint64_t)t * NP * GN + lc) * P;
-    AT acc = AT(0);
 #pragma unroll
     for (int j = 0; j < NP; ++j) {
         KPack<KT, P> k;
@@ -194,15 +256,16 @@ __device__ __forceinline__ VT ebe_tile_acc(const KT* __restrict__ K, const uint3
 }
 
 // one tile, stored by its own lanes
-template <int N, typename KT, typename VT, bool MASKED, bool ATOMIC>
+template <int N, typename KT, typename VT, bool MASKED, bool ATOMIC, bool SYM = false>
 __device__ __forceinline__ void ebe_tile_body(const KT* __restrict__ K, const uint32_t* __restrict__ map,
                                               const VT* __restrict__ u, VT* __restrict__ out, uint64_t ve_base,
-                                              uint32_t rs, uint32_t ss, uint32_t t, const float* __restrict__ kdiag)
+                                              uint32_t rs, uint32_t ss, uint32_t t, const float* __restrict__ kdiag,
+                                              KT* ksh = nullptr)
 {
     constexpr int G = 64 / N;
     constexpr int GN = G * N;
     uint32_t m;
-    const VT acc = ebe_tile_acc<N, KT, VT, MASKED>(K, map, u, t, kdiag, m);
+    const VT acc = ebe_tile_acc<N, KT, VT, MASKED, SYM>(K, map, u, t, kdiag, m, ksh);
     const uint32_t lane = threadIdx.x & 63u;
     if (lane < (uint32_t)GN) {
         if constexpr (ATOMIC) {
@@ -220,7 +283,7 @@ __device__ __forceinline__ void ebe_tile_body(const KT* __restrict__ K, const ui
 // flight together).  Multi-element tiles with the deterministic scatter (rows
 // stored [r][slot]) stage the block's rows in LDS and store each row index's
 // consecutive slots together.
-template <int N, typename KT, typename VT, bool MASKED, bool ATOMIC, int TPW>
+template <int N, typename KT, typename VT, bool MASKED, bool ATOMIC, int TPW, bool SYM = false>
 __global__ __launch_bounds__(256) void ebe_tile_kernel(const KT* __restrict__ K, const uint32_t* __restrict__ map,
                                                       const VT* __restrict__ u, VT* __restrict__ out,
                                                       uint64_t ve_base, uint32_t rs, uint32_t ss, uint32_t tile_begin,
@@ -228,12 +291,16 @@ __global__ __launch_bounds__(256) void ebe_tile_kernel(const KT* __restrict__ K,
                                                       const float* __restrict__ kdiag)
 {
     constexpr int G = 64 / N;
+    constexpr uint32_t TE = sym_region_values<N, KT>();
+    // packed symmetric tiles: one LDS region per tile of the block
+    __shared__ __align__(16) KT ksh[SYM ? 4u * TPW * TE : 1u];
     if (skip && *skip) return;  // converged solver: nothing to do
     const uint32_t w = threadIdx.x >> 6;
     if constexpr (G == 1 || ATOMIC) {
         const uint32_t tb = xcd_block(blockIdx.x, gridDim.x) * 4u;
         if (tb + w >= n_tiles) return;  // uniform per wave
-        ebe_tile_body<N, KT, VT, MASKED, ATOMIC>(K, map, u, out, ve_base, rs, ss, tile_begin + tb + w, kdiag);
+        ebe_tile_body<N, KT, VT, MASKED, ATOMIC, SYM>(K, map, u, out, ve_base, rs, ss, tile_begin + tb + w, kdiag,
+                                                      ksh + (SYM ? w * TE : 0u));
     } else {
         constexpr int GN = G * N;
         constexpr uint32_t BT = 4u * TPW;  // tiles per block
@@ -246,12 +313,14 @@ __global__ __launch_bounds__(256) void ebe_tile_kernel(const KT* __restrict__ K,
         if (nt == BT) {
 #pragma unroll
             for (int i = 0; i < TPW; i++)
-                acc[i] = ebe_tile_acc<N, KT, VT, MASKED>(K, map, u, tile_begin + tb + w * TPW + i, kdiag, m);
+                acc[i] = ebe_tile_acc<N, KT, VT, MASKED, SYM>(K, map, u, tile_begin + tb + w * TPW + i, kdiag, m,
+                                                              ksh + (SYM ? (w * TPW + i) * TE : 0u));
         } else {
 #pragma unroll
             for (int i = 0; i < TPW; i++)
                 if (w * TPW + i < nt)
-                    acc[i] = ebe_tile_acc<N, KT, VT, MASKED>(K, map, u, tile_begin + tb + w * TPW + i, kdiag, m);
+                    acc[i] = ebe_tile_acc<N, KT, VT, MASKED, SYM>(K, map, u, tile_begin + tb + w * TPW + i, kdiag, m,
+                                                                  ksh + (SYM ? (w * TPW + i) * TE : 0u));
         }
         if (lane < (uint32_t)GN) {
             const uint32_t g = lane / N, r = lane - g * N;
@@ -297,12 +366,28 @@ template <int N, typename KT, typename VT>
 __global__ __launch_bounds__(256) void ebe_tile_batched_kernel(const EbeJob* __restrict__ jobs,
                                                               const uint32_t* __restrict__ prefix, uint32_t n_jobs)
 {
+    constexpr bool SYM_OK = sym_region_values<N, KT>() > 1u;
+    __shared__ __align__(16) KT ksh[SYM_OK ? 4u * sym_region_values<N, KT>() : 1u];
     const uint32_t gt = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * 4u + (threadIdx.x >> 6));
     if (gt >= prefix[n_jobs]) return;
     const uint32_t j = owner_of(prefix, n_jobs, gt);
     const EbeJob& jb = jobs[j];
     if (jb.skip && *jb.skip) return;
     const uint32_t t = jb.tile_begin + (gt - prefix[j]);
+    if constexpr (SYM_OK) {
+        if (jb.sym) {
+            KT* reg = ksh + (threadIdx.x >> 6) * sym_region_values<N, KT>();
+            if (jb.masked)
+                ebe_tile_body<N, KT, VT, true, false, true>(static_cast<const KT*>(jb.K), jb.map,
+                                                            static_cast<const VT*>(jb.u), static_cast<VT*>(jb.out),
+                                                            jb.ve_base, jb.ve_rs, jb.ve_ss, t, jb.kdiag, reg);
+            else
+                ebe_tile_body<N, KT, VT, false, false, true>(static_cast<const KT*>(jb.K), jb.map,
+                                                             static_cast<const VT*>(jb.u), static_cast<VT*>(jb.out),
+                                                             jb.ve_base, jb.ve_rs, jb.ve_ss, t, jb.kdiag, reg);
+            return;
+        }
+    }
     if (jb.masked)
         ebe_tile_body<N, KT, VT, true, false>(static_cast<const KT*>(jb.K), jb.map, static_cast<const VT*>(jb.u),
                                               static_cast<VT*>(jb.out), jb.ve_base, jb.ve_rs, jb.ve_ss, t, jb.kdiag);
@@ -478,6 +563,33 @@ __global__ void relayout_scaled_kernel(const double* __restrict__ kref, const do
     dst[idx] = from_f64<KT>(val);
 }
 
+// packed symmetric tiles from row-major matrices (src) or k_ref + scale: one
+// thread per stored value, padding included (zeros)
+template <typename KT>
+__global__ void relayout_sym_kernel(const double* __restrict__ src, const double* __restrict__ kref,
+                                    const double* __restrict__ scale, const uint32_t* __restrict__ slot_elem,
+                                    uint64_t total, uint32_t N, uint32_t G, uint32_t TE, KT* __restrict__ dst)
+{
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const uint64_t t = idx / TE;
+    const uint32_t w = (uint32_t)(idx - t * TE), NS = sym_ns(N);
+    double val = 0.0;
+    if (w < G * NS) {
+        const uint32_t g = w / NS;
+        uint32_t k = w - g * NS, a = 0;
+        while (k >= N - a) {  // row a of the upper triangle holds N - a values
+            k -= N - a;
+            a++;
+        }
+        const uint32_t b = a + k;
+        const uint32_t e = slot_elem[t * G + g];
+        if (e != kPadSlot && !(std::is_same<KT, uint16_t>::value && a == b))  // BF16: diagonal kept in FP32
+            val = src ? src[(uint64_t)e * N * N + (uint64_t)a * N + b] : scale[e] * kref[(uint64_t)a * N + b];
+    }
+    dst[idx] = from_f64<KT>(val);
+}
+
 // FP32 diagonal of the BF16 storage: d_r = K_rr + sum_{j != r} (K_rj - bf16(K_rj)),
 // so that every row of the stored matrix sums to the FP64 row sum (constants
 // stay in the null space of a Laplacian element); symmetric K stays symmetric.
@@ -502,8 +614,8 @@ __global__ void bf16_diag_kernel(const double* __restrict__ src, const double* _
     out[i] = (float)d;
 }
 
-__global__ void extract_diag_kernel(const double* __restrict__ K, TiledIndex ti, uint64_t n_rows, uint32_t rs,
-                                    uint32_t ss, double* __restrict__ ve)
+__global__ void extract_diag_kernel(const double* __restrict__ K, TiledIndex ti, uint32_t sym_te, uint64_t n_rows,
+                                    uint32_t rs, uint32_t ss, double* __restrict__ ve)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;  // slot * N + r
     if (i >= n_rows) return;
@@ -512,7 +624,8 @@ __global__ void extract_diag_kernel(const double* __restrict__ K, TiledIndex ti,
     const uint64_t t = slot / ti.G;
     const uint32_t g = (uint32_t)(slot - t * ti.G);
     const uint32_t GN = ti.G * ti.N, NP = ti.N / ti.P;
-    const uint64_t k = ((t * NP + r / ti.P) * GN + g * ti.N + r) * ti.P + r % ti.P;
+    const uint64_t k = sym_te ? t * sym_te + g * sym_ns(ti.N) + sym_idx(ti.N, r, r)
+                              : ((t * NP + r / ti.P) * GN + g * ti.N + r) * ti.P + r % ti.P;
     ve[(uint64_t)r * rs + slot * ss] = K[k];
 }
 
@@ -569,7 +682,7 @@ __global__ void f2d_kernel(const float* __restrict__ a, double* __restrict__ b, 
 inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
 
 template <int N, typename KT, typename VT>
-void launch_tile(bool masked, bool atomic, const EbeArgs& a, hipStream_t s)
+void launch_tile(bool masked, bool atomic, bool sym, const EbeArgs& a, hipStream_t s)
 {
     constexpr int TPW = tiles_per_wave<N, KT>();
     const uint32_t per_block = atomic ? 4u : 4u * TPW;
@@ -577,15 +690,28 @@ void launch_tile(bool masked, bool atomic, const EbeArgs& a, hipStream_t s)
     const KT* K = static_cast<const KT*>(a.K);
     const VT* u = static_cast<const VT*>(a.u);
     VT* out = static_cast<VT*>(a.out);
-#define AMX_TILE(M, A, T)                                                                                          \
-    ebe_tile_kernel<N, KT, VT, M, A, T><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.ve_rs, a.ve_ss,      \
-                                                                a.tile_begin, a.n_tiles, a.skip, a.kdiag)
+#define AMX_TILE(M, A, T, S)                                                                                       \
+    ebe_tile_kernel<N, KT, VT, M, A, T, S><<<grid, block, 0, s>>>(K, a.map, u, out, a.ve_base, a.ve_rs, a.ve_ss,   \
+                                                                   a.tile_begin, a.n_tiles, a.skip, a.kdiag)
+    if constexpr (N == 8 || N == 10) {  // packed symmetric tiles (sym_supported)
+        if (sym) {
+            if (masked) {
+                if (atomic) AMX_TILE(true, true, 1, true);
+                else AMX_TILE(true, false, TPW, true);
+            } else {
+                if (atomic) AMX_TILE(false, true, 1, true);
+                else AMX_TILE(false, false, TPW, true);
+            }
+            return;
+        }
+    }
+    AMX_REQUIRE(!sym, AMX_INVALID_STATE, "no packed symmetric tile kernel for n = " + std::to_string(N));
     if (masked) {
-        if (atomic) AMX_TILE(true, true, 1);
-        else AMX_TILE(true, false, TPW);
+        if (atomic) AMX_TILE(true, true, 1, false);
+        else AMX_TILE(true, false, TPW, false);
     } else {
-        if (atomic) AMX_TILE(false, true, 1);
-        else AMX_TILE(false, false, TPW);
+        if (atomic) AMX_TILE(false, true, 1, false);
+        else AMX_TILE(false, false, TPW, false);
     }
 #undef AMX_TILE
 }
@@ -610,15 +736,16 @@ template <typename KT, typename VT>
 void dispatch_n(const Layout& L, bool masked, bool atomic, const EbeArgs& a, hipStream_t s)
 {
     if (!L.specialized) return launch_generic<KT, VT>(L.n, masked, atomic, a, s);
+    const bool sy = L.sym;
     switch (L.n) {
-    case 4: return launch_tile<4, KT, VT>(masked, atomic, a, s);
-    case 8: return launch_tile<8, KT, VT>(masked, atomic, a, s);
-    case 10: return launch_tile<10, KT, VT>(masked, atomic, a, s);
-    case 12: return launch_tile<12, KT, VT>(masked, atomic, a, s);
-    case 20: return launch_tile<20, KT, VT>(masked, atomic, a, s);
-    case 24: return launch_tile<24, KT, VT>(masked, atomic, a, s);
-    case 30: return launch_tile<30, KT, VT>(masked, atomic, a, s);
-    case 60: return launch_tile<60, KT, VT>(masked, atomic, a, s);
+    case 4: return launch_tile<4, KT, VT>(masked, atomic, sy, a, s);
+    case 8: return launch_tile<8, KT, VT>(masked, atomic, sy, a, s);
+    case 10: return launch_tile<10, KT, VT>(masked, atomic, sy, a, s);
+    case 12: return launch_tile<12, KT, VT>(masked, atomic, sy, a, s);
+    case 20: return launch_tile<20, KT, VT>(masked, atomic, sy, a, s);
+    case 24: return launch_tile<24, KT, VT>(masked, atomic, sy, a, s);
+    case 30: return launch_tile<30, KT, VT>(masked, atomic, sy, a, s);
+    case 60: return launch_tile<60, KT, VT>(masked, atomic, sy, a, s);
     default: fail(AMX_INVALID_STATE, "no tile kernel for n = " + std::to_string(L.n));
     }
 }
@@ -634,6 +761,14 @@ bool has_tile_kernel(uint32_t n)
 }  // namespace
 
 size_t ktype_size(KType kt) { return kt == KType::F64 ? 8 : kt == KType::F32 ? 4 : 2; }
+
+bool sym_supported(uint32_t n) { return n == 8 || n == 10; }
+
+uint64_t tile_values(const Layout& L, KType kt)
+{
+    if (L.sym) return sym_tile_values(L.n, (uint32_t)ktype_size(kt));
+    return (uint64_t)L.G * L.n * L.n;
+}
 
 Layout choose_layout(uint32_t n, KType kt)
 {
@@ -653,9 +788,25 @@ Layout choose_layout(uint32_t n, KType kt)
     return L;
 }
 
+void launch_relayout_sym(const double* src, const double* k_ref, const double* scale, const uint32_t* slot_elem,
+                         uint32_t n_slots, const Layout& L, KType kt, void* dst, hipStream_t s)
+{
+    const uint32_t TE = (uint32_t)tile_values(L, kt);
+    const uint64_t total = (uint64_t)(n_slots / L.G) * TE;
+    if (total == 0) return;
+    const uint32_t nb = blocks_for(total, 256);
+    switch (kt) {
+    case KType::F64: relayout_sym_kernel<double><<<nb, 256, 0, s>>>(src, k_ref, scale, slot_elem, total, L.n, L.G, TE, (double*)dst); break;
+    case KType::F32: relayout_sym_kernel<float><<<nb, 256, 0, s>>>(src, k_ref, scale, slot_elem, total, L.n, L.G, TE, (float*)dst); break;
+    case KType::BF16: relayout_sym_kernel<uint16_t><<<nb, 256, 0, s>>>(src, k_ref, scale, slot_elem, total, L.n, L.G, TE, (uint16_t*)dst); break;
+    }
+    AMX_HIP(hipGetLastError());
+}
+
 void launch_relayout_from_rowmajor(const double* src, const uint32_t* slot_elem, uint32_t n_slots, const Layout& L,
                                    KType kt, void* dst, hipStream_t s)
 {
+    if (L.sym) return launch_relayout_sym(src, nullptr, nullptr, slot_elem, n_slots, L, kt, dst, s);
     const uint64_t total = (uint64_t)n_slots * L.n * L.n;
     if (total == 0) return;
     const TiledIndex ti{L.n, L.G, L.P};
@@ -671,6 +822,7 @@ void launch_relayout_from_rowmajor(const double* src, const uint32_t* slot_elem,
 void launch_relayout_scaled(const double* k_ref, const double* scale, const uint32_t* slot_elem, uint32_t n_slots,
                             const Layout& L, KType kt, void* dst, hipStream_t s)
 {
+    if (L.sym) return launch_relayout_sym(nullptr, k_ref, scale, slot_elem, n_slots, L, kt, dst, s);
     const uint64_t total = (uint64_t)n_slots * L.n * L.n;
     if (total == 0) return;
     const TiledIndex ti{L.n, L.G, L.P};
@@ -697,7 +849,8 @@ void launch_extract_diag(const double* K, const Layout& L, uint32_t n_slots, uin
 {
     const uint64_t rows = (uint64_t)n_slots * L.n;
     if (rows == 0) return;
-    extract_diag_kernel<<<blocks_for(rows, 256), 256, 0, s>>>(K, TiledIndex{L.n, L.G, L.P}, rows, rs, ss, ve);
+    const uint32_t te = L.sym ? (uint32_t)tile_values(L, KType::F64) : 0u;
+    extract_diag_kernel<<<blocks_for(rows, 256), 256, 0, s>>>(K, TiledIndex{L.n, L.G, L.P}, te, rows, rs, ss, ve);
     AMX_HIP(hipGetLastError());
 }
 

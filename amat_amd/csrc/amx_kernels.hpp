@@ -16,13 +16,25 @@ enum class KType : int { F64 = 0, F32 = 1, BF16 = 2 };
 // j*P .. j*P+P-1 of row r, so one pack is one contiguous G*n*P-value load of
 // the whole wave.  Offset of (slot, r, c), slot = t*G + g:
 //   ((t*(n/P) + c/P)*(G*n) + g*n + r)*P + c%P.
+//
+// Packed symmetric tiles (sym): when every element matrix of a group is
+// bitwise symmetric, a multi-element tile stores only the upper triangles:
+// element g of tile t at t*TE + g*NS, NS = n(n+1)/2 values, entry (a <= b) at
+// a*n - a(a-1)/2 + (b - a) (row-major upper triangle); TE = G*NS rounded up
+// to 16 bytes.  The kernels read K[r][c] = K[min][max]: the same values in the
+// same order as the full layout, about half the bytes.
 struct Layout {
     uint32_t n = 0, G = 1, P = 1;
     bool specialized = false;  // a template kernel exists for (n, G, P, type)
+    bool sym = false;          // packed symmetric tiles
 };
 
 Layout choose_layout(uint32_t n, KType kt);
 size_t ktype_size(KType kt);
+// packed symmetric tiles exist for these DoF counts (multi-element tiles)
+bool sym_supported(uint32_t n);
+// values per stored tile of layout L in storage kt (padding included)
+uint64_t tile_values(const Layout& L, KType kt);
 
 // ---- set-up kernels -----------------------------------------------------
 // row-major FP64 element matrices (group element order) -> tiled layout of kt.
@@ -77,7 +89,7 @@ struct EbeJob {
     uint32_t tile_begin, n_tiles;
     const uint32_t* skip;   // the job's converged flag
     const float* kdiag;
-    uint32_t masked, pad;
+    uint32_t masked, sym;   // sym: packed symmetric tiles
 };
 struct GatherJob {
     const uint32_t* off;

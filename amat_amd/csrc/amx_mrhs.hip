@@ -547,8 +547,8 @@ __global__ __launch_bounds__(256) void mrhs_gather_kernel(const uint32_t* __rest
 // 60 DoFs -> compact MFMA order (pos60); smaller -> FP32 column-major n x n or
 // BF16 row-major n x ld.  One thread per stored value.
 __global__ void mrhs_layout_kernel(const double* __restrict__ K64, uint32_t N, uint32_t G, uint32_t P,
-                                   const uint32_t* __restrict__ slot_of, uint32_t n_elem, int bf16, uint32_t ld,
-                                   void* __restrict__ out)
+                                   uint32_t sym_te, const uint32_t* __restrict__ slot_of, uint32_t n_elem, int bf16,
+                                   uint32_t ld, void* __restrict__ out)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t per = (bf16 && N != 60) ? (uint64_t)N * ld : (uint64_t)N * N;
@@ -573,7 +573,12 @@ __global__ void mrhs_layout_kernel(const double* __restrict__ K64, uint32_t N, u
         const uint64_t slot = slot_of[e];
         const uint64_t t = slot / G;
         const uint32_t g = (uint32_t)(slot - t * G), GN = G * N, NP = N / P;
-        val = (float)K64[((t * NP + c / P) * GN + g * N + r) * P + c % P];
+        if (sym_te) {  // packed symmetric tiles: K[r][c] = K[min][max] of the upper triangle
+            const uint32_t a = r < c ? r : c, b = r < c ? c : r;
+            val = (float)K64[t * sym_te + g * (N * (N + 1) / 2) + a * N - (a * (a - 1)) / 2 + (b - a)];
+        } else {
+            val = (float)K64[((t * NP + c / P) * GN + g * N + r) * P + c % P];
+        }
     }
     if (bf16) static_cast<__bf16*>(out)[dst] = (__bf16)val;
     else static_cast<float*>(out)[dst] = val;
@@ -784,7 +789,9 @@ int amx_mrhs_create(amx_op* const* ops, int n_ops, int precision, amx_mrhs** out
             p.K.alloc(std::max<uint64_t>(total, 8) * (m->bf16 ? 2 : 4));
             if (total)
                 amx::mrhs_layout_kernel<<<(uint32_t)((total + 255) / 256), 256, 0, op.stream>>>(
-                    g.K[0].as<double>(), g.n, g.L[0].G, g.L[0].P, dslot.as<uint32_t>(), p.n_elem, m->bf16 ? 1 : 0, ld,
+                    g.K[0].as<double>(), g.n, g.L[0].G, g.L[0].P,
+                    g.L[0].sym ? (uint32_t)amx::tile_values(g.L[0], amx::KType::F64) : 0u, dslot.as<uint32_t>(),
+                    p.n_elem, m->bf16 ? 1 : 0, ld,
                     p.K.p);
             AMX_HIP(hipGetLastError());
             AMX_HIP(hipStreamSynchronize(op.stream));

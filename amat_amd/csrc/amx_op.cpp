@@ -205,6 +205,24 @@ void finalize_op(Op& op)
         g.L[1] = choose_layout(n, KType::F32);
         g.L[2] = choose_layout(n, KType::BF16);
         g.G = g.L[0].G;
+        // packed symmetric tiles when every stored matrix of the group is
+        // bitwise symmetric (then K[r][c] and K[c][r] are the same value)
+        if (!free_m && op.sym_mode == 1 && g.G > 1 && sym_supported(n)) {
+            bool sym = true;
+            auto same = [](double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; };
+            if (op.scaled) {
+                for (uint32_t r = 0; r < n && sym; r++)
+                    for (uint32_t c = r + 1; c < n && sym; c++)
+                        sym = same(op.kref[(size_t)r * n + c], op.kref[(size_t)c * n + r]);
+            } else if (!g.stage.empty()) {
+                for (size_t i = 0; i < g.elems.size() && sym; i++) {
+                    const double* ke = g.stage.data() + i * n * n;
+                    for (uint32_t r = 0; r < n && sym; r++)
+                        for (uint32_t c = r + 1; c < n && sym; c++) sym = same(ke[(size_t)r * n + c], ke[(size_t)c * n + r]);
+                }
+            }
+            for (Layout& L : g.L) L.sym = sym;
+        }
         // independent elements first, then dependent, each padded to a tile
         g.slot_elem.clear();
         g.elem_slot.assign(g.elems.size(), kPadSlot);
@@ -262,7 +280,7 @@ void finalize_op(Op& op)
         const int bits[3] = {AMX_PREC_FP64, AMX_PREC_FP32, AMX_PREC_BF16};
         for (int kt = 0; kt < 3; kt++) {
             if (kt > 0 && !op.has(bits[kt])) continue;
-            const uint64_t bytes = (uint64_t)g.n_slots * n * n * ktype_size((KType)kt);
+            const uint64_t bytes = (uint64_t)g.n_tiles * tile_values(g.L[kt], (KType)kt) * ktype_size((KType)kt);
             g.K[kt].alloc(std::max<uint64_t>(bytes, 16));
             op.device_bytes += g.K[kt].bytes;
             if (op.scaled)
@@ -753,6 +771,16 @@ int amx_op_set_options(amx_op* h, int precision_mask, int scatter_mode)
     });
 }
 
+int amx_op_set_symmetric_storage(amx_op* h, int mode)
+{
+    return guarded(__func__, [&] {
+        AMX_REQUIRE(h, AMX_INVALID_ARGUMENT, "op is NULL");
+        AMX_REQUIRE(mode == 0 || mode == 1, AMX_INVALID_ARGUMENT, "mode: 0 (full tiles) or 1 (packed when symmetric)");
+        if (mode != h->op.sym_mode) h->op.finalized = false;
+        h->op.sym_mode = mode;
+    });
+}
+
 int amx_op_set_penalty(amx_op* h, double p)
 {
     return guarded(__func__, [&] {
@@ -814,10 +842,13 @@ int amx_op_get_info(const amx_op* h, amx_op_info* info)
                 info->flops += ne * amx::free_flops_per_elem(op.elem_kind, op.elem_nq);
                 continue;
             }
-            info->k_bytes_fp64 += ne * nn * 8;
-            info->k_bytes_fp32 += ne * nn * 4;
-            info->k_bytes_bf16 += ne * nn * 2;
-            info->alg_bytes_fp64 += ne * (nn * 8 + 4ull * g.n);
+            // values stored per element: the upper triangle of packed symmetric tiles
+            const uint64_t ns = g.L[0].sym ? (uint64_t)g.n * (g.n + 1) / 2 : nn;
+            if (g.L[0].sym) info->packed_symmetric++;
+            info->k_bytes_fp64 += ne * ns * 8;
+            info->k_bytes_fp32 += ne * ns * 4;
+            info->k_bytes_bf16 += ne * ns * 2;
+            info->alg_bytes_fp64 += ne * (ns * 8 + 4ull * g.n);
             info->flops += 2 * ne * nn;
         }
         info->alg_bytes_fp64 += 16ull * op.n_total;

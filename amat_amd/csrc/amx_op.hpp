@@ -44,6 +44,7 @@ struct Op {
     int bc_operator = AMX_BCOP_IDENTITY;
     int scatter = AMX_SCATTER_DETERMINISTIC;
     int prec_mask = AMX_PREC_FP64;
+    int sym_mode = 1;                    // packed symmetric tiles: 0 never, 1 when every matrix is symmetric
     bool penalty_set = false;
     double penalty_value = 0.0;
     double trace = 0.0;

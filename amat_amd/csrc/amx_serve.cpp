@@ -183,6 +183,7 @@ void rebuild_tables(amx_server& S)
             a.skip = skip;
             a.kdiag = j.kt == KType::BF16 ? grp.kdiag16.as<float>() : nullptr;
             a.masked = op.bc_operator == AMX_BCOP_IDENTITY ? 1u : 0u;
+            a.sym = grp.L[(int)j.kt].sym ? 1u : 0u;
             e.first.push_back(a);
             e.second.push_back(e.second.back() + grp.n_tiles);
         }

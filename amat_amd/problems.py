@@ -60,10 +60,13 @@ class Problem:
         m.set_bdr_map(self.constrained, self.prescribed)
         return m
 
-    def build_operator(self, aMatFree, Maps, bc_meth=0, precision=1, scatter=0, device=0, bcop=None, penalty=None):
+    def build_operator(self, aMatFree, Maps, bc_meth=0, precision=1, scatter=0, device=0, bcop=None, penalty=None,
+                       sym_storage=None):
         maps = self.build_maps(Maps)
         op = aMatFree(maps, bc_meth, device)
         op.set_matfree_type(3)
+        if sym_storage is not None:
+            op.set_symmetric_storage(sym_storage)
         if self.K is not None:
             op.set_element_matrices(0, self.K)
         else:

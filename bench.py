@@ -170,15 +170,20 @@ def precision_mode(args, lite: bool = False) -> dict:
     t0 = time.time()
     p = problems.poisson_tet10(args.n, seed=3, jitter=0.2)
     log(f"[precision] tet10 {args.n}^3 x 6: {p.n_elem} elements, {p.n_dofs} DoFs ({time.time() - t0:.1f}s)")
-    op = p.build_operator(aMatFree, Maps, precision=PREC.FP64 | PREC.FP32 | PREC.BF16, bcop=BCOP.IDENTITY)
+    op = p.build_operator(aMatFree, Maps, precision=PREC.FP64 | PREC.FP32 | PREC.BF16, bcop=BCOP.IDENTITY,
+                          sym_storage=args.sym_storage)
     del p.K
     log(f"[precision] operator finalized ({time.time() - t0:.1f}s)")
     b = op.apply_bc(p.rhs)
     db = torch.from_numpy(b).cuda()
     s = torch.cuda.current_stream()
-    res = {"metric": "precision: IR-CG vs FP64 CG to 1e-10, jittered tet10 Poisson", "elements": p.n_elem, "dofs": p.n_dofs}
-    # matvec rates per storage
+    inf = op.info()
+    res = {"metric": "precision: IR-CG vs FP64 CG to 1e-10, jittered tet10 Poisson", "elements": p.n_elem, "dofs": p.n_dofs,
+           "packed_symmetric": bool(inf.packed_symmetric)}
+    # matvec rates per storage; algorithmic bytes of the stored form (n^2 values
+    # per element, or n(n+1)/2 for packed symmetric tiles)
     E, n, N = p.n_elem, 10, p.n_dofs
+    nv = n * (n + 1) // 2 if inf.packed_symmetric else n * n
     u64 = torch.from_numpy(p.random_u()).cuda()
     v64 = torch.empty_like(u64)
     u32 = u64.float()
@@ -188,9 +193,10 @@ def precision_mode(args, lite: bool = False) -> dict:
                                    ("bf16", lambda a, b_, st: op.matmult_f32_dev(PREC.BF16, a, b_, st), v32, u32, 2, 4)):
         ms = time_matvec(fn, v, u, args.steps, args.warmup, torch)
         diag_bytes = E * n * 4 if name == "bf16" else 0
-        alg = E * (n * n * sk + 4 * n) + diag_bytes + N * 2 * sv
+        alg = E * (nv * sk + 4 * n) + diag_bytes + N * 2 * sv
+        full = E * (n * n * sk + 4 * n) + diag_bytes + N * 2 * sv
         res[f"matvec_{name}"] = {"ms": ms, "GDoF/s": N / ms / 1e6, "hbm_frac": alg / (ms * 1e-3) / HBM_PEAK,
-                                 "alg_bytes": alg}
+                                 "alg_bytes": alg, "whole_matrix_bytes": full}
         log(f"[precision] matvec {name}: {ms:.3f} ms, {N / ms / 1e6:.2f} GDoF/s")
     x64 = torch.zeros_like(db)
     torch.cuda.synchronize()
@@ -582,6 +588,9 @@ def main() -> int:
     ap.add_argument("--batch-tiles", default="",
                     help="rhs_tile,waves,split: a fixed launch shape instead of the autotuner (profiling passes)")
     ap.add_argument("--inner-rtol", type=float, default=1e-2)
+    ap.add_argument("--sym-storage", type=int, choices=[0, 1], default=None,
+                    help="element-matrix storage of 8-/10-DoF groups in --mode precision: 0 whole, 1 packed when "
+                         "symmetric (default: the library's, 1)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)

@@ -207,8 +207,11 @@ typedef struct amx_op_info {
     double penalty;             /* value placed on constrained diagonals */
     uint64_t device_bytes;      /* device memory held by the handle */
     uint64_t k_bytes_fp64, k_bytes_fp32, k_bytes_bf16; /* element-matrix storage */
-    uint64_t alg_bytes_fp64;    /* E(n^2 8 + 4n) + 16 N: one FP64 matvec's algorithmic HBM bytes */
+    uint64_t alg_bytes_fp64;    /* E(s 8 + 4n) + 16 N, s = values stored per element (n^2, or n(n+1)/2
+                                   packed): one FP64 matvec's algorithmic HBM bytes */
     uint64_t flops;             /* 2 sum_e n_e^2 per matvec */
+    uint32_t packed_symmetric;  /* element groups stored as packed symmetric tiles */
+    uint32_t reserved;
 } amx_op_info;
 
 /* aMatFree(Maps&, BC_METH) (include/aMatFree.hpp:405-439).  Copies the maps it
@@ -236,6 +239,14 @@ int amx_op_set_element_function(amx_op* op, int kind, const double* xe, const do
 /* aMatFree::set_num_streams (include/aMatFree.hpp:230-237): number of HIP
  * streams used by the two-phase (independent/dependent) launch; >= 1 */
 int amx_op_set_num_streams(amx_op* op, uint32_t n_streams);
+
+/* Storage of the element matrices of 8- and 10-DoF groups (no counterpart in
+ * the reference, which stores every matrix whole, include/aMatFree.hpp:882-953):
+ * mode 1 (default) packs the upper triangles when every matrix of the group is
+ * bitwise symmetric -- the kernels then read K[r][c] = K[min(r,c)][max(r,c)]:
+ * the same values in the same order, about half the bytes; mode 0 keeps whole
+ * matrices.  Takes effect at the next finalize. */
+int amx_op_set_symmetric_storage(amx_op* op, int mode);
 
 /* aMatFree::set_element_matrix -> copy_element_matrix
  * (include/aMatFree.hpp:854-953): block (block_i, block_j) of element eid,

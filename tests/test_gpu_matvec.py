@@ -358,3 +358,47 @@ def test_measured_read_bandwidth(torch_gpu):
     assert 1e12 < bw < 1e13
     with pytest.raises(AmatError):
         measure_read_bandwidth(0, 1024, 1)
+
+
+@pytest.mark.parametrize("which", ["hex8", "tet10"])
+def test_packed_symmetric_storage(torch_gpu, which):
+    """8- and 10-DoF element matrices that are bitwise symmetric are stored as
+    packed upper triangles (about half the bytes); the kernels read the same
+    values in the same order, so every product is bit-identical to the whole
+    storage (FP64, FP32, BF16; deterministic scatter) and the FP64 one to the
+    oracle.  One asymmetric entry makes the group fall back to whole matrices."""
+    p = {"hex8": lambda: problems.poisson_hex8(7, 6, 5), "tet10": lambda: problems.poisson_tet10(3)}[which]()
+    K = p.element_matrices().copy()
+
+    def make(Kset, mode, scatter=SCATTER.DETERMINISTIC):
+        op = aMatFree(p.build_maps(Maps))
+        op.set_matfree_type(3)
+        op.set_element_matrices(0, Kset)
+        op.set_options(PREC.FP64 | PREC.FP32 | PREC.BF16, scatter)
+        op.set_symmetric_storage(mode)
+        op.finalize()
+        return op
+
+    packed, whole = make(K, 1), make(K, 0)
+    ip, iw = packed.info(), whole.info()
+    assert ip.packed_symmetric == 1 and iw.packed_symmetric == 0
+    n = p.dofs_per_elem
+    assert ip.k_bytes_fp64 * n * n == iw.k_bytes_fp64 * n * (n + 1) // 2
+    u = p.random_u()
+    for fn in ("matvec_dev", "matmult_dev"):
+        a = run(torch_gpu, getattr(packed, fn), u)
+        np.testing.assert_array_equal(a, run(torch_gpu, getattr(whole, fn), u))
+    np.testing.assert_array_equal(run(torch_gpu, packed.matvec_dev, u), orc.matvec(p.gmap, K, u))
+    for prec in (PREC.FP32, PREC.BF16):
+        a = run(torch_gpu, lambda v, uu, s: packed.matmult_f32_dev(prec, v, uu, s), u, np.float32)
+        b = run(torch_gpu, lambda v, uu, s: whole.matmult_f32_dev(prec, v, uu, s), u, np.float32)
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(packed.get_diagonal(), whole.get_diagonal())
+    ato = make(K, 1, SCATTER.ATOMIC)
+    assert rel_l2(run(torch_gpu, ato.matmult_dev, u), run(torch_gpu, whole.matmult_dev, u)) <= 1e-13
+    # one entry off symmetry: whole matrices, still the oracle's product
+    K2 = K.copy()
+    K2[1, 0, n - 1] = np.nextafter(K2[1, 0, n - 1], 1.0)
+    fb = make(K2, 1)
+    assert fb.info().packed_symmetric == 0
+    np.testing.assert_array_equal(run(torch_gpu, fb.matvec_dev, u), orc.matvec(p.gmap, K2, u))
