@@ -129,6 +129,7 @@ SIGNATURES = {
     "amx_op_synchronize": (I32, [P]),
     "amx_cg_solve_dev": (I32, [P, P, P, F64, I32, C.POINTER(SolveStats), P]),
     "amx_cg_solve_noalloc_dev": (I32, [P, P, P, F64, I32, C.POINTER(SolveStats), P]),
+    "amx_dev_dot2": (I32, [P, P, U64, U32, U32, P, I32, P]),
     "amx_ir_cg_solve_dev": (I32, [P, I32, P, P, F64, F64, I32, I32, C.POINTER(SolveStats), P]),
     "amx_mrhs_create": (I32, [P, I32, I32, P]),
     "amx_mrhs_destroy": (I32, [P]),

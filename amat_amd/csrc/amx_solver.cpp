@@ -362,4 +362,28 @@ int amx_ir_cg_solve_dev(amx_op* h, int precision, const double* b, double* x, do
     });
 }
 
+int amx_dev_dot2(const double* x, const double* y, uint64_t n, uint32_t n_blocks, uint32_t repeat, double* out,
+                 int device, void* stream)
+{
+    return guarded(__func__, [&] {
+        AMX_REQUIRE((x && y) || n == 0, AMX_INVALID_ARGUMENT, "NULL argument");
+        AMX_REQUIRE(out, AMX_INVALID_ARGUMENT, "NULL argument");
+        AMX_REQUIRE(n <= 0xffffffffull, AMX_INVALID_ARGUMENT, "n exceeds 2^32 - 1");
+        AMX_REQUIRE(n_blocks >= 1 && n_blocks <= amx::kMaxRedBlocks, AMX_INVALID_ARGUMENT,
+                    "n_blocks must be in [1, 1024]");
+        AMX_REQUIRE(repeat >= 1, AMX_INVALID_ARGUMENT, "repeat must be positive");
+        amx::DeviceGuard dg(device);
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        // partials and group counters, then the top counter, all zero
+        DevBuf ws;
+        ws.alloc(amx::kPartialsBytes + 256);
+        AMX_HIP(hipMemsetAsync(ws.p, 0, ws.bytes, s));
+        double* partials = ws.as<double>();
+        uint32_t* counter = reinterpret_cast<uint32_t*>(ws.as<char>() + amx::kPartialsBytes);
+        for (uint32_t k = 0; k < repeat; k++)
+            amx::dot2(x, y, (uint32_t)n, n_blocks, partials, counter, out + 2 * (size_t)k, s);
+        AMX_HIP(hipStreamSynchronize(s));
+    });
+}
+
 }  // extern "C"

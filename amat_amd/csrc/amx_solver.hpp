@@ -55,6 +55,10 @@ void correct(double* x, const float* d32, uint32_t n, const CgScalars* sc, hipSt
 void scale_to_f64(const double* r, double* rs, uint32_t n, const CgScalars* sc, hipStream_t s);
 void correct64(double* x, const double* d, uint32_t n, const CgScalars* sc, hipStream_t s);
 void invert(const double* d, double* dinv, uint32_t n, hipStream_t s);
+// out[0..1] = x.y, y.y (FP64) by the solvers' reduction over n_blocks blocks
+// (partials: kPartialsBytes with zero counters; counter: zero, left at zero)
+void dot2(const double* x, const double* y, uint32_t n, uint32_t n_blocks, double* partials, uint32_t* counter,
+          double* out, hipStream_t s);
 // batched iteration pieces: prefix[j] = first block of job j (prefix[n_jobs] = total_blocks)
 void cg_pq_batched(bool f64, const CgJob* jobs, const uint32_t* prefix, uint32_t n_jobs, uint32_t total_blocks,
                    hipStream_t s);

@@ -280,6 +280,26 @@ __global__ void correct_kernel(double* __restrict__ x, const VT* __restrict__ d,
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) x[i] += s * (double)d[i];
 }
 
+// x.y and y.y over the solvers' reduction (the element-to-thread assignment
+// and summation order of cg_pq_kernel), result to out[0..1]
+__global__ __launch_bounds__(kBlock) void dot2_kernel(const double* __restrict__ x, const double* __restrict__ y,
+                                                     uint32_t n, double* partials, uint32_t* counter,
+                                                     double* __restrict__ out)
+{
+    __shared__ double sh[kBlock];
+    double acc[2] = {0.0, 0.0};
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        acc[0] += x[i] * y[i];
+        acc[1] += y[i] * y[i];
+    }
+    double tot[2];
+    if (!reduce_partials<2>(acc, partials, counter, tot, sh)) return;
+    if (threadIdx.x == 0) {
+        out[0] = tot[0];
+        out[1] = tot[1];
+    }
+}
+
 __global__ void invert_kernel(const double* __restrict__ d, double* __restrict__ dinv, uint32_t n)
 {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -583,6 +603,13 @@ void ru_replace(bool f64, bool init, const double* b, const double* ax, void* rs
 void cg_reset(CgScalars* sc, double rtol, uint32_t max_it, hipStream_t s, double delta, uint32_t max_upd)
 {
     cg_reset_kernel<<<1, 64, 0, s>>>(sc, rtol * rtol, max_it, delta * delta, max_upd);
+    AMX_HIP(hipGetLastError());
+}
+
+void dot2(const double* x, const double* y, uint32_t n, uint32_t n_blocks, double* partials, uint32_t* counter,
+          double* out, hipStream_t s)
+{
+    dot2_kernel<<<n_blocks, kBlock, 0, s>>>(x, y, n, partials, counter, out);
     AMX_HIP(hipGetLastError());
 }
 

@@ -443,6 +443,18 @@ int amx_ir_cg_solve_dev(amx_op* op, int precision, const double* b, double* x, d
                         double inner_rtol, int max_outer, int max_inner,
                         amx_solve_stats* stats, void* stream);
 
+/* Self-check of the solvers' deterministic inner products (no counterpart in
+ * the reference).  out (device, 2 * repeat doubles): out[2k] = x.y and
+ * out[2k+1] = y.y in FP64, computed `repeat` times in a row by the reduction
+ * every CG kernel uses, over n_blocks (1..1024) blocks of 256 threads: thread
+ * t of block b sums elements i = b*256 + t, then i + n_blocks*256, ... in
+ * ascending order; a tree over each block's 256 sums (stride 128, 64, ... 1);
+ * the blocks in groups of 32 (the same tree over a group's sums, zero-padded);
+ * the same tree over the group sums.  x, y: device arrays of n FP64 values on
+ * `device`.  Synchronizes `stream`. */
+int amx_dev_dot2(const double* x, const double* y, uint64_t n, uint32_t n_blocks, uint32_t repeat,
+                 double* out, int device, void* stream);
+
 /* ------------------------------------------------ multi-RHS element products
  * V_i = sum_e P_e^T K_e P_e U_i for several operators i in ONE persistent
  * launch, each U_i / V_i a device array of n_total x nrhs floats (DoF-major,

@@ -1,6 +1,8 @@
 """Device solvers: FP64 Jacobi-PCG against a direct solve and the oracle's
 PCG; FP64 iterative refinement with FP32 / BF16 inner solves reaching the
 same 1e-10 relative residual, with their distance to the FP64 solution."""
+import math
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -151,3 +153,89 @@ def test_fp64_cg_residual_replacement(torch_gpu):
     x.zero_()
     st3 = op.cg(db, x, rtol=1e-10, max_it=20000)
     assert st3.converged and st3.outer_iterations == 0 and st3.rel_residual <= 1e-10
+
+
+def _tree(a):
+    """The block tree of the device reduction: a[i] += a[i + s] for s = 128 .. 1."""
+    a = a.copy()
+    s = a.shape[-1] // 2
+    while s > 0:
+        a = a[..., :s] + a[..., s:2 * s]
+        s //= 2
+    return a[..., 0]
+
+
+def host_dot_fixed_order(x, n_blocks):
+    """sum(x) in the summation order of the solvers' reduction over n_blocks
+    blocks of 256 threads: per-thread strided sums, block trees, group trees
+    (32 blocks, zero-padded to 256), the tree over the group sums."""
+    S = n_blocks * 256
+    rows = max(1, -(-len(x) // S))
+    xs = np.zeros(rows * S)
+    xs[:len(x)] = x
+    acc = np.zeros(S)
+    for k in range(rows):
+        acc = acc + xs[k * S:(k + 1) * S]
+    b = _tree(acc.reshape(n_blocks, 256))
+    n_groups = -(-n_blocks // 32)
+    gs = np.zeros(n_groups)
+    for g in range(n_groups):
+        pad = np.zeros(256)
+        part = b[32 * g:32 * (g + 1)]
+        pad[:len(part)] = part
+        gs[g] = _tree(pad)
+    pad = np.zeros(256)
+    pad[:n_groups] = gs
+    return _tree(pad)
+
+
+def test_reduction_fixed_order_sweep(torch_gpu):
+    """The fence-free cross-block reduction of the CG kernels, checked bit for
+    bit against a host sum in the same fixed order, for sizes around block and
+    group boundaries and grids of 1 to 1024 blocks, each reduction repeated 16
+    times in a row on one partials buffer (every repeat equal): a deposit seen
+    late by the finishing block, or a counter left non-zero, changes a sum.
+    y = 1, so x.y is sum(x) and y.y = n exactly whatever the contraction of
+    the products.  Stops at the first mismatch."""
+    import ctypes as C
+
+    from amat_amd._capi import lib
+
+    torch = torch_gpu
+    L = lib()
+    rng = np.random.default_rng(11)
+    repeat = 16
+    for n in [1, 255, 256, 257, 8191, 32768, 65537, 262143, 1 << 20, (1 << 22) + 17]:
+        x = rng.standard_normal(n) * np.exp(rng.uniform(-20, 20, n))  # wide exponents: order-sensitive sums
+        xd = dev(torch, x)
+        yd = torch.ones(n, dtype=torch.float64, device="cuda")
+        for nb in [1, 2, 31, 32, 33, 64, 100, 257, 1000, 1024]:
+            out = torch.full((2 * repeat,), np.nan, dtype=torch.float64, device="cuda")
+            rc = L.amx_dev_dot2(C.c_void_p(xd.data_ptr()), C.c_void_p(yd.data_ptr()), n, nb, repeat,
+                                C.c_void_p(out.data_ptr()), 0, None)
+            assert rc == 0
+            got = out.cpu().numpy().reshape(repeat, 2)
+            want = host_dot_fixed_order(x, nb)
+            bad = [k for k in range(repeat) if got[k, 0] != want or got[k, 1] != float(n)]
+            assert not bad, (n, nb, bad[:4], got[bad[0]].tolist(), want)
+            # and the fixed-order sum is a sum: close to the compensated one
+            assert abs(want - math.fsum(x)) <= 1e-10 * np.abs(x).sum(), (n, nb)
+
+
+def test_reduction_rejects_bad_arguments(torch_gpu):
+    import ctypes as C
+
+    from amat_amd._capi import lib
+
+    torch = torch_gpu
+    L = lib()
+    xd = torch.ones(10, dtype=torch.float64, device="cuda")
+    out = torch.zeros(2, dtype=torch.float64, device="cuda")
+    p = C.c_void_p(xd.data_ptr())
+    o = C.c_void_p(out.data_ptr())
+    assert L.amx_dev_dot2(p, p, 10, 0, 1, o, 0, None) != 0
+    assert L.amx_dev_dot2(p, p, 10, 1025, 1, o, 0, None) != 0
+    assert L.amx_dev_dot2(p, p, 10, 4, 0, o, 0, None) != 0
+    assert L.amx_dev_dot2(p, p, 10, 4, 1, None, 0, None) != 0
+    assert L.amx_dev_dot2(p, p, 10, 4, 1, o, 0, None) == 0
+    assert out.cpu().tolist() == [10.0, 10.0]
