@@ -259,14 +259,24 @@ class Done:
 class ServingLoop:
     """Admission by HBM budget + continuous batching on one Server.
 
-    Each turn: admit arrived requests (FIFO) while a slot is free and the
-    operator + workspace fit the budget; run one step of the server; retire
-    finished jobs.  Latency = finish - arrival; service time = finish -
-    admission (the job on the GPU); queue wait = admission - arrival."""
+    Each turn: admit arrived requests while a slot is free and the operator +
+    workspace fit the budget; run one step of the server; retire finished
+    jobs.  Latency = finish - arrival; service time = finish - admission (the
+    job on the GPU); queue wait = admission - arrival.
+
+    Admission order: "fifo" takes the oldest request (head of line: one that
+    does not fit waits).  "affinity" takes the oldest request among the first
+    `window` waiting ones whose operator is resident, so a request that would
+    build (and evict) an operator lets those that reuse one go first; no
+    request is overtaken more than `max_bypass` times (the scan stops at a
+    request that has been), and with every operator resident it is FIFO."""
 
     def __init__(self, server: Server, cache: OperatorCache, method: PREC, budget_bytes: int, n_iter: int = 8,
                  rtol: float = 1e-10, inner_rtol: float = 1e-2, keep_x: bool = False,
-                 clock: Callable[[], float] = time.perf_counter):
+                 clock: Callable[[], float] = time.perf_counter, admission: str = "fifo", window: int = 256,
+                 max_bypass: int = 64):
+        if admission not in ("fifo", "affinity"):
+            raise ValueError(f"admission must be 'fifo' or 'affinity', not {admission!r}")
         self.server = server
         self.cache = cache
         self.method = int(method)
@@ -276,6 +286,19 @@ class ServingLoop:
         self.inner_rtol = inner_rtol
         self.keep_x = keep_x
         self.clock = clock
+        self.admission = admission
+        self.window = max(1, int(window))
+        self.max_bypass = max(0, int(max_bypass))
+
+    def _pick(self, pending, bypassed: dict) -> int:
+        """Index in `pending` of the request to admit next."""
+        if self.admission == "fifo":
+            return 0
+        for j in range(min(self.window, len(pending))):
+            r = pending[j]
+            if (r.kind, r.n) in self.cache.items or bypassed.get(r.rid, 0) >= self.max_bypass:
+                return j
+        return 0
 
     def run(self, requests: list[Request], realtime: bool = True,
             progress: Callable[[float, int, int, int], None] | None = None) -> dict:
@@ -290,14 +313,16 @@ class ServingLoop:
         peak_jobs = 0
         peak_bytes = 0
         builds0, evict0, build_s0 = self.cache.builds, self.cache.evictions, self.cache.build_seconds
-        budget_waits = 0   # turns on which the head request waited for HBM (a slot was free)
+        budget_waits = 0   # turns on which the chosen request waited for HBM (a slot was free)
+        bypassed: dict[int, int] = {}  # request id -> times a later request was admitted first
         while incoming or pending or running:
             now = self.clock() - t0
             while incoming and (incoming[0].arrival <= now or not realtime):
                 pending.append(incoming.popleft())
-            # admission (FIFO, head-of-line: a request that does not fit waits)
+            # admission: the picked request waits (and so does everything behind it) if it does not fit
             while pending and len(running) < self.server.max_jobs:
-                r = pending[0]
+                idx = self._pick(pending, bypassed)
+                r = pending[idx]
                 key = (r.kind, r.n)
                 c = self.cache.get(key)
                 need_op = 0 if c is not None else self.cache.estimate(r.kind, r.n)
@@ -315,7 +340,10 @@ class ServingLoop:
                 jb = self.server.job_bytes(c.op, self.method)
                 workspace += jb
                 running[jid] = (r, c, self.clock() - t0, jb)
-                pending.popleft()
+                for j in range(idx):
+                    bypassed[pending[j].rid] = bypassed.get(pending[j].rid, 0) + 1
+                del pending[idx]
+                bypassed.pop(r.rid, None)
             peak_jobs = max(peak_jobs, len(running))
             peak_bytes = max(peak_bytes, workspace + self.cache.bytes)
             if progress is not None:
@@ -353,6 +381,7 @@ class ServingLoop:
             "operator_build_s": self.cache.build_seconds - build_s0,
             "evictions": self.cache.evictions - evict0,
             "budget_waits": budget_waits,
+            "admission": self.admission,
             "all_converged": all(d.converged for d in done),
             "max_rel_residual": max((d.rel_residual for d in done), default=0.0),
             "peak_jobs": peak_jobs,

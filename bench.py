@@ -479,7 +479,8 @@ def serve_mode(args) -> dict:
     1e-10 relative residual) arriving as Poisson traffic, served by the
     continuous-batching server under an HBM budget; FP64 CG and FP64-refined
     FP32/BF16 CG on the same request streams.  Operators are built once
-    (resident cache) before the timed streams."""
+    (resident cache) before the timed streams, or, with --serve-cold, inside
+    them on first use (and again after an eviction when the budget binds)."""
     import numpy as np
     from amat_amd import PREC, serve
     from amat_amd.amat import IR_RELIABLE
@@ -512,13 +513,15 @@ def serve_mode(args) -> dict:
     res = {"metric": "served CG solves/s and latency at 1e-10 relative residual", "requests": args.serve_requests,
            "rates": rates, "budget_GiB": args.serve_budget_gb, "slots": args.serve_slots, "step_iters": args.serve_iters,
            "inner_rtol": args.inner_rtol, "ru_delta": args.ru_delta, "cold_start": bool(args.serve_cold),
+           "admission": args.serve_admission,
            "operators": len(keys), "operator_GiB": cache.bytes / 2**30, "operator_build_s": cache.build_seconds,
            "runs": []}
     for rate in rates:
         for name in names:
             server = serve.Server(max_jobs=args.serve_slots)
             loop = serve.ServingLoop(server, caches[name.endswith("free")], methods[name], budget, n_iter=args.serve_iters,
-                                     inner_rtol=args.ru_delta if name in reliable else args.inner_rtol)
+                                     inner_rtol=args.ru_delta if name in reliable else args.inner_rtol,
+                                     admission=args.serve_admission)
             last = [0.0]
 
             def prog(el, fin, run, pend):
@@ -579,6 +582,8 @@ def main() -> int:
     ap.add_argument("--serve-cold", action="store_true",
                     help="start with no resident operator: builds, admission waits and evictions happen in the stream")
     ap.add_argument("--serve-slots", type=int, default=64)
+    ap.add_argument("--serve-admission", choices=["fifo", "affinity"], default="fifo",
+                    help="admission order: oldest first, or oldest with a resident operator first (bounded bypass)")
     ap.add_argument("--serve-iters", type=int, default=8)
     ap.add_argument("--serve-seed", type=int, default=7)
     ap.add_argument("--batch-n", type=int, default=48, help="hex20 elements per edge in --mode batch")

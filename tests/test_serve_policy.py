@@ -134,3 +134,50 @@ def test_estimates_cover_small_meshes():
         E, nd = p.gmap.shape
         assert serve.estimate_op_bytes(kind, n) >= E * nd * nd * 14
         assert serve.estimate_job_bytes(kind, n, PREC.FP64) >= 7 * p.n_dofs * 8
+
+
+def _run_policy(monkeypatch, admission, max_bypass=8, n_req=200, budget=12_000, warm=False):
+    monkeypatch.setattr(serve, "rhs_for", lambda p, seed: np.zeros(p.n_dofs))
+    monkeypatch.setattr(serve, "estimate_op_bytes", lambda kind, n, free=False: 1000 * n)
+    monkeypatch.setattr(serve, "estimate_job_bytes", lambda kind, n, m: 100)
+    log = []
+    cache = serve.OperatorCache(builder=fake_builder(log))
+    srv = FakeServer(max_jobs=3)
+    reqs = serve.make_requests(n_req, rate=1e9, seed=5, mix=(("hex8", 1.0, 1, 9),))
+    if warm:
+        for key in sorted({(r.kind, r.n) for r in reqs}):
+            cache.build(key)
+    loop = serve.ServingLoop(srv, cache, PREC.FP64, budget, clock=Clock(), admission=admission,
+                             max_bypass=max_bypass)
+    return loop.run(reqs, realtime=False), cache
+
+
+def test_affinity_admission_builds_less_and_bounds_overtaking(monkeypatch):
+    fifo, cf = _run_policy(monkeypatch, "fifo")
+    aff, ca = _run_policy(monkeypatch, "affinity", max_bypass=8)
+    assert fifo["solves"] == aff["solves"] == 200 and aff["all_converged"]
+    assert aff["admission"] == "affinity" and fifo["admission"] == "fifo"
+    assert aff["peak_resident_bytes"] <= 12_000 and aff["peak_jobs"] <= 3
+    # requests that reuse a resident operator go first: fewer builds and evictions than FIFO
+    assert ca.builds < cf.builds and ca.evictions < cf.evictions
+    # no request is overtaken by more than max_bypass later requests (arrival order = rid order here)
+    order = [d.rid for d in sorted(aff["done"], key=lambda d: (d.admitted, d.rid))]
+    pos = {rid: i for i, rid in enumerate(order)}
+    for rid in pos:
+        overtook = sum(1 for o in pos if o > rid and pos[o] < pos[rid])
+        assert overtook <= 8, (rid, overtook)
+    # FIFO admits strictly in arrival order
+    forder = [d.rid for d in sorted(fifo["done"], key=lambda d: (d.admitted, d.rid))]
+    assert forder == sorted(forder)
+
+
+def test_affinity_is_fifo_when_every_operator_is_resident(monkeypatch):
+    fifo, _ = _run_policy(monkeypatch, "fifo", budget=10 ** 9, warm=True)
+    aff, _ = _run_policy(monkeypatch, "affinity", budget=10 ** 9, warm=True)
+    assert [d.rid for d in aff["done"]] == [d.rid for d in fifo["done"]]
+    assert aff["operator_builds"] == fifo["operator_builds"]
+
+
+def test_unknown_admission_is_rejected():
+    with pytest.raises(ValueError):
+        serve.ServingLoop(FakeServer(1), serve.OperatorCache(builder=fake_builder([])), PREC.FP64, 1, admission="lifo")
