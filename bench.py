@@ -621,6 +621,8 @@ def main() -> int:
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     if args.mode == "precision":
         print(json.dumps(precision_mode(args)), flush=True)
+        if args.calibrate_copy:
+            calibrate_copy(torch)
         return 0
     if args.mode == "batch":
         print(json.dumps(batch_mode(args)), flush=True)
