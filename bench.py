@@ -452,9 +452,14 @@ def extras(args) -> dict:
     torch.cuda.empty_cache()
     keep = ("seconds", "iterations", "inner", "outer", "rel_residual", "converged", "rel_error_vs_fp64_solution",
             "speedup_vs_fp64_cg")
+    # hbm_frac: algorithmic bytes of the stored form (packed triangles when
+    # packed_symmetric) over the whole matvec; whole_matrix_frac: the same time
+    # against the bytes of whole n x n element matrices
     out["precision_tet10_96"] = {
-        "elements": pr["elements"], "dofs": pr["dofs"],
-        "matvec": {k[7:]: {"ms": v["ms"], "hbm_frac": v["hbm_frac"]} for k, v in pr.items() if k.startswith("matvec_")},
+        "elements": pr["elements"], "dofs": pr["dofs"], "packed_symmetric": pr["packed_symmetric"],
+        "matvec": {k[7:]: {"ms": v["ms"], "hbm_frac": v["hbm_frac"], "alg_bytes": v["alg_bytes"],
+                           "whole_matrix_frac": v["whole_matrix_bytes"] / (v["ms"] * 1e-3) / HBM_PEAK}
+                   for k, v in pr.items() if k.startswith("matvec_")},
         **{k: {f: v[f] for f in keep if f in v} for k, v in pr.items() if k.startswith(("cg_", "ru_cg"))},
         "seconds_in_bench": time.time() - t0}
     t0 = time.time()
