@@ -19,6 +19,7 @@ Poisson n ~ U{12..48}; each request has its own seeded RHS in [-1, 1)
 from __future__ import annotations
 
 import collections
+import concurrent.futures as cf
 import ctypes as C
 import dataclasses
 import math
@@ -203,14 +204,23 @@ class OperatorCache:
                 self.evictions += 1
         return self.bytes + other + need <= budget
 
-    def build(self, key) -> CachedOp:
+    def make(self, key) -> tuple:
+        """Build the operator of `key` without touching the cache (safe on a
+        worker thread): (problem, operator, device bytes, seconds)."""
         t0 = time.perf_counter()
         p, op, nbytes = self.builder(*key)
-        self.build_seconds += time.perf_counter() - t0
+        return p, op, nbytes, time.perf_counter() - t0
+
+    def add(self, key, made: tuple) -> CachedOp:
+        p, op, nbytes, seconds = made
+        self.build_seconds += seconds
         self.builds += 1
         c = CachedOp(key, p, op, nbytes)
         self.items[key] = c
         return c
+
+    def build(self, key) -> CachedOp:
+        return self.add(key, self.make(key))
 
 
 def estimate_op_bytes(kind: str, n: int, free: bool = False) -> int:
@@ -269,12 +279,18 @@ class ServingLoop:
     `window` waiting ones whose operator is resident, so a request that would
     build (and evict) an operator lets those that reuse one go first; no
     request is overtaken more than `max_bypass` times (the scan stops at a
-    request that has been), and with every operator resident it is FIFO."""
+    request that has been), and with every operator resident it is FIFO.
+
+    build_workers > 0: an operator that must be built is built on that many
+    worker threads while the server keeps stepping the running jobs (its
+    bytes are reserved in the budget from the start); the request waiting for
+    it is admitted when it is ready.  0: built inline, the server idle
+    meanwhile.  Results do not depend on it (a job's arithmetic is its own)."""
 
     def __init__(self, server: Server, cache: OperatorCache, method: PREC, budget_bytes: int, n_iter: int = 8,
                  rtol: float = 1e-10, inner_rtol: float = 1e-2, keep_x: bool = False,
                  clock: Callable[[], float] = time.perf_counter, admission: str = "fifo", window: int = 256,
-                 max_bypass: int = 64):
+                 max_bypass: int = 64, build_workers: int = 0):
         if admission not in ("fifo", "affinity"):
             raise ValueError(f"admission must be 'fifo' or 'affinity', not {admission!r}")
         self.server = server
@@ -289,6 +305,7 @@ class ServingLoop:
         self.admission = admission
         self.window = max(1, int(window))
         self.max_bypass = max(0, int(max_bypass))
+        self.build_workers = max(0, int(build_workers))
 
     def _pick(self, pending, bypassed: dict) -> int:
         """Index in `pending` of the request to admit next."""
@@ -315,50 +332,70 @@ class ServingLoop:
         builds0, evict0, build_s0 = self.cache.builds, self.cache.evictions, self.cache.build_seconds
         budget_waits = 0   # turns on which the chosen request waited for HBM (a slot was free)
         bypassed: dict[int, int] = {}  # request id -> times a later request was admitted first
-        while incoming or pending or running:
-            now = self.clock() - t0
-            while incoming and (incoming[0].arrival <= now or not realtime):
-                pending.append(incoming.popleft())
-            # admission: the picked request waits (and so does everything behind it) if it does not fit
-            while pending and len(running) < self.server.max_jobs:
-                idx = self._pick(pending, bypassed)
-                r = pending[idx]
-                key = (r.kind, r.n)
-                c = self.cache.get(key)
-                need_op = 0 if c is not None else self.cache.estimate(r.kind, r.n)
-                need_job = estimate_job_bytes(r.kind, r.n, self.method)
-                if not self.cache.evict_for(need_op + need_job, self.budget, workspace):
-                    if not running:
-                        raise RuntimeError(f"request {r.rid} ({r.kind} n={r.n}) cannot fit the HBM budget")
-                    budget_waits += 1
-                    break
-                if c is None:
-                    c = self.cache.build(key)
-                b = rhs_for(c.problem, r.seed)
-                jid = self.server.submit(c.op, self.method, b, rtol=self.rtol, inner_rtol=self.inner_rtol)
-                c.users += 1
-                jb = self.server.job_bytes(c.op, self.method)
-                workspace += jb
-                running[jid] = (r, c, self.clock() - t0, jb)
-                for j in range(idx):
-                    bypassed[pending[j].rid] = bypassed.get(pending[j].rid, 0) + 1
-                del pending[idx]
-                bypassed.pop(r.rid, None)
-            peak_jobs = max(peak_jobs, len(running))
-            peak_bytes = max(peak_bytes, workspace + self.cache.bytes)
-            if progress is not None:
-                progress(self.clock() - t0, len(done), len(running), len(pending) + len(incoming))
-            if running:
-                self.server.step(self.n_iter)
-                for jid in self.server.poll():
-                    r, c, t_adm, jb = running.pop(jid)
-                    x, st = self.server.result(jid, c.problem.n_dofs if self.keep_x else None)
-                    c.users -= 1
-                    workspace -= jb
-                    done.append(Done(r.rid, r.kind, r.n, r.arrival, t_adm, self.clock() - t0, st.iterations,
-                                     st.outer_iterations, st.rel_residual, bool(st.converged), x))
-            elif incoming and realtime:
-                time.sleep(max(0.0, min(incoming[0].arrival - (self.clock() - t0), 0.01)))
+        pool = cf.ThreadPoolExecutor(max_workers=self.build_workers) if self.build_workers else None
+        building: dict[tuple, tuple[cf.Future, int]] = {}  # key -> (build, bytes reserved for it)
+        reserved = 0
+        try:
+            while incoming or pending or running or building:
+                now = self.clock() - t0
+                while incoming and (incoming[0].arrival <= now or not realtime):
+                    pending.append(incoming.popleft())
+                # operators built in the background join the cache (on this thread)
+                for key in [k for k, (f, _) in building.items() if f.done()]:
+                    f, nb = building.pop(key)
+                    reserved -= nb
+                    self.cache.add(key, f.result())
+                # admission: the picked request waits (and so does everything behind it) if it does not fit
+                while pending and len(running) < self.server.max_jobs:
+                    idx = self._pick(pending, bypassed)
+                    r = pending[idx]
+                    key = (r.kind, r.n)
+                    c = self.cache.get(key)
+                    if c is None and key in building:
+                        break  # its operator is being built
+                    need_op = 0 if c is not None else self.cache.estimate(r.kind, r.n)
+                    need_job = estimate_job_bytes(r.kind, r.n, self.method)
+                    if not self.cache.evict_for(need_op + need_job, self.budget, workspace + reserved):
+                        if not running and not building:
+                            raise RuntimeError(f"request {r.rid} ({r.kind} n={r.n}) cannot fit the HBM budget")
+                        budget_waits += 1
+                        break
+                    if c is None:
+                        if pool is not None:
+                            building[key] = (pool.submit(self.cache.make, key), need_op)
+                            reserved += need_op
+                            break
+                        c = self.cache.build(key)
+                    b = rhs_for(c.problem, r.seed)
+                    jid = self.server.submit(c.op, self.method, b, rtol=self.rtol, inner_rtol=self.inner_rtol)
+                    c.users += 1
+                    jb = self.server.job_bytes(c.op, self.method)
+                    workspace += jb
+                    running[jid] = (r, c, self.clock() - t0, jb)
+                    for j in range(idx):
+                        bypassed[pending[j].rid] = bypassed.get(pending[j].rid, 0) + 1
+                    del pending[idx]
+                    bypassed.pop(r.rid, None)
+                peak_jobs = max(peak_jobs, len(running))
+                peak_bytes = max(peak_bytes, workspace + self.cache.bytes + reserved)
+                if progress is not None:
+                    progress(self.clock() - t0, len(done), len(running), len(pending) + len(incoming))
+                if running:
+                    self.server.step(self.n_iter)
+                    for jid in self.server.poll():
+                        r, c, t_adm, jb = running.pop(jid)
+                        x, st = self.server.result(jid, c.problem.n_dofs if self.keep_x else None)
+                        c.users -= 1
+                        workspace -= jb
+                        done.append(Done(r.rid, r.kind, r.n, r.arrival, t_adm, self.clock() - t0, st.iterations,
+                                         st.outer_iterations, st.rel_residual, bool(st.converged), x))
+                elif building:
+                    cf.wait([f for f, _ in building.values()], timeout=0.01, return_when=cf.FIRST_COMPLETED)
+                elif incoming and realtime:
+                    time.sleep(max(0.0, min(incoming[0].arrival - (self.clock() - t0), 0.01)))
+        finally:
+            if pool is not None:
+                pool.shutdown(wait=True)
         makespan = self.clock() - t0
         lat = np.array([d.finished - d.arrival for d in done])
         svc = np.array([d.finished - d.admitted for d in done])
@@ -382,6 +419,7 @@ class ServingLoop:
             "evictions": self.cache.evictions - evict0,
             "budget_waits": budget_waits,
             "admission": self.admission,
+            "build_workers": self.build_workers,
             "all_converged": all(d.converged for d in done),
             "max_rel_residual": max((d.rel_residual for d in done), default=0.0),
             "peak_jobs": peak_jobs,

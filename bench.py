@@ -518,7 +518,7 @@ def serve_mode(args) -> dict:
     res = {"metric": "served CG solves/s and latency at 1e-10 relative residual", "requests": args.serve_requests,
            "rates": rates, "budget_GiB": args.serve_budget_gb, "slots": args.serve_slots, "step_iters": args.serve_iters,
            "inner_rtol": args.inner_rtol, "ru_delta": args.ru_delta, "cold_start": bool(args.serve_cold),
-           "admission": args.serve_admission,
+           "admission": args.serve_admission, "build_workers": args.serve_build_workers,
            "operators": len(keys), "operator_GiB": cache.bytes / 2**30, "operator_build_s": cache.build_seconds,
            "runs": []}
     for rate in rates:
@@ -526,7 +526,7 @@ def serve_mode(args) -> dict:
             server = serve.Server(max_jobs=args.serve_slots)
             loop = serve.ServingLoop(server, caches[name.endswith("free")], methods[name], budget, n_iter=args.serve_iters,
                                      inner_rtol=args.ru_delta if name in reliable else args.inner_rtol,
-                                     admission=args.serve_admission)
+                                     admission=args.serve_admission, build_workers=args.serve_build_workers)
             last = [0.0]
 
             def prog(el, fin, run, pend):
@@ -589,6 +589,8 @@ def main() -> int:
     ap.add_argument("--serve-slots", type=int, default=64)
     ap.add_argument("--serve-admission", choices=["fifo", "affinity"], default="fifo",
                     help="admission order: oldest first, or oldest with a resident operator first (bounded bypass)")
+    ap.add_argument("--serve-build-workers", type=int, default=0,
+                    help="threads building operators while the server steps (0: built inline)")
     ap.add_argument("--serve-iters", type=int, default=8)
     ap.add_argument("--serve-seed", type=int, default=7)
     ap.add_argument("--batch-n", type=int, default=48, help="hex20 elements per edge in --mode batch")

@@ -21,6 +21,7 @@ class FakeServer:
         self.finished = []
         self.next = 1
         self.active_max = 0
+        self.steps = 0
 
     @staticmethod
     def job_bytes(op, method):
@@ -35,6 +36,7 @@ class FakeServer:
         return jid
 
     def step(self, n_iter):
+        self.steps += 1
         for j in list(self.jobs):
             self.jobs[j] -= 1
             if self.jobs[j] == 0:
@@ -181,3 +183,39 @@ def test_affinity_is_fifo_when_every_operator_is_resident(monkeypatch):
 def test_unknown_admission_is_rejected():
     with pytest.raises(ValueError):
         serve.ServingLoop(FakeServer(1), serve.OperatorCache(builder=fake_builder([])), PREC.FP64, 1, admission="lifo")
+
+
+@pytest.mark.parametrize("admission", ["fifo", "affinity"])
+def test_background_builds_overlap_the_server(monkeypatch, admission):
+    """With build workers the server keeps stepping while an operator is
+    built; every request is served, within the slots and the budget (the
+    bytes of an operator under construction are reserved)."""
+    import time as _time
+    monkeypatch.setattr(serve, "rhs_for", lambda p, seed: np.zeros(p.n_dofs))
+    monkeypatch.setattr(serve, "estimate_op_bytes", lambda kind, n, free=False: 1000 * n)
+    monkeypatch.setattr(serve, "estimate_job_bytes", lambda kind, n, m: 100)
+    class SlowServer(FakeServer):
+        def step(self, n_iter):   # a step takes GPU time
+            _time.sleep(0.001)
+            super().step(n_iter)
+
+    srv = SlowServer(max_jobs=3, steps_per_job=20)
+    overlap = []
+
+    def slow_build(kind, n):
+        s0 = srv.steps
+        _time.sleep(0.01)
+        overlap.append(srv.steps - s0)
+        return FakeProblem(), {"bytes_job": 100}, 1000 * n
+
+    cache = serve.OperatorCache(builder=slow_build)
+    reqs = serve.make_requests(60, rate=1e9, seed=3, mix=(("hex8", 1.0, 1, 6),))
+    loop = serve.ServingLoop(srv, cache, PREC.FP64, 16_000, clock=Clock(), admission=admission, build_workers=2)
+    m = loop.run(reqs, realtime=False)
+    assert m["solves"] == 60 and m["all_converged"] and m["build_workers"] == 2
+    assert m["peak_resident_bytes"] <= 16_000 and srv.active_max <= 3
+    assert m["operator_builds"] == len(overlap) == cache.builds > 0
+    assert any(k > 0 for k in overlap)   # the server stepped during some build
+    if admission == "fifo":
+        order = [d.rid for d in sorted(m["done"], key=lambda d: (d.admitted, d.rid))]
+        assert order == sorted(order)
