@@ -543,6 +543,38 @@ __global__ __launch_bounds__(256) void mrhs_gather_kernel(const uint32_t* __rest
     V[i] = s;
 }
 
+// the same sums, four columns per thread (nrhs a multiple of 4, 16-byte
+// aligned rows): one 16-byte load per entry; each column's sum keeps the order
+// of mrhs_gather_kernel, so the results are the same bits
+__global__ __launch_bounds__(256) void mrhs_gather4_kernel(const uint32_t* __restrict__ off,
+                                                          const uint32_t* __restrict__ pos,
+                                                          const float4* __restrict__ ve, float4* __restrict__ V,
+                                                          uint32_t n_total, uint32_t nq)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)n_total * nq) return;
+    const uint32_t d = (uint32_t)(i / nq), c = (uint32_t)(i - (uint64_t)d * nq);
+    const uint32_t b = off[d], cnt = off[d + 1] - b;
+    float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (uint32_t q0 = 0; q0 < cnt; q0 += 8) {
+        uint32_t p[8];
+        float4 x[8];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; j++) p[j] = pos[b + min(q0 + j, cnt - 1)];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; j++) x[j] = ve[(uint64_t)p[j] * nq + c];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; j++)
+            if (q0 + j < cnt) {
+                s.x += x[j].x;
+                s.y += x[j].y;
+                s.z += x[j].z;
+                s.w += x[j].w;
+            }
+    }
+    V[i] = s;
+}
+
 // element matrices for the MFMA kernels from the tiled FP64 storage:
 // 60 DoFs -> compact MFMA order (pos60); smaller -> FP32 column-major n x n or
 // BF16 row-major n x ld.  One thread per stored value.
@@ -696,9 +728,16 @@ void launch(amx_mrhs& m, int nrhs, const float* const* U, float* const* V, hipSt
             auto& p = m.parts[i];
             const uint64_t len = (uint64_t)p.n_total * nrhs;
             if (!len) continue;
-            mrhs_gather_kernel<<<(uint32_t)((len + 255) / 256), 256, 0, s>>>(
-                p.off.as<uint32_t>(), p.pos.as<uint32_t>(), m.ve.as<float>() + p.ve_row0 * (uint64_t)nrhs, V[i],
-                p.n_total, (uint32_t)nrhs);
+            const float* ve = m.ve.as<float>() + p.ve_row0 * (uint64_t)nrhs;
+            if (nrhs % 4 == 0 && reinterpret_cast<uintptr_t>(V[i]) % 16 == 0 &&
+                reinterpret_cast<uintptr_t>(ve) % 16 == 0) {
+                mrhs_gather4_kernel<<<(uint32_t)((len / 4 + 255) / 256), 256, 0, s>>>(
+                    p.off.as<uint32_t>(), p.pos.as<uint32_t>(), reinterpret_cast<const float4*>(ve),
+                    reinterpret_cast<float4*>(V[i]), p.n_total, (uint32_t)(nrhs / 4));
+            } else {
+                mrhs_gather_kernel<<<(uint32_t)((len + 255) / 256), 256, 0, s>>>(
+                    p.off.as<uint32_t>(), p.pos.as<uint32_t>(), ve, V[i], p.n_total, (uint32_t)nrhs);
+            }
         }
         AMX_HIP(hipGetLastError());
     }
