@@ -58,7 +58,7 @@ class OpInfo(C.Structure):
         (n, C.c_int32) for n in ("matfree_type", "bc_meth", "bc_operator", "scatter_mode", "precision_mask")] + [
         ("trace", C.c_double), ("penalty", C.c_double)] + [
         (n, C.c_uint64) for n in ("device_bytes", "k_bytes_fp64", "k_bytes_fp32", "k_bytes_bf16", "alg_bytes_fp64",
-                                  "flops")] + [("packed_symmetric", C.c_uint32), ("reserved", C.c_uint32)]
+                                  "flops")] + [("packed_symmetric", C.c_uint32), ("near_symmetric", C.c_uint32)]
 
 
 class SolveStats(C.Structure):

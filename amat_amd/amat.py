@@ -384,7 +384,9 @@ class aMatFree:
 
     def set_symmetric_storage(self, mode: int):
         """1 (default): pack the upper triangles of 8-/10-DoF element matrices
-        when all are bitwise symmetric; 0: whole matrices (next finalize)."""
+        when all are bitwise symmetric, and keep 60-DoF FP64 matrices that are
+        symmetric up to low-word round-off as near-symmetric records (every
+        entry rebuilt bit for bit); 0: whole matrices (next finalize)."""
         call("amx_op_set_symmetric_storage", self._h, int(mode))
 
     def set_penalty(self, p: float):

@@ -21,6 +21,7 @@
 
 #include "amx_internal.hpp"
 #include "amx_kernels.hpp"
+#include "amx_nsym.hpp"
 
 namespace amx {
 
@@ -416,6 +417,125 @@ template <int N, typename KT> constexpr int tiles_per_wave()
 // four: 0.486 and 0.405), one for FP64 (two: 0.788 against 0.781)
 template <int N, typename KT> constexpr int sym_tiles_per_wave() { return KSize<KT>::v == 8 ? 1 : 2; }
 
+// ------------------------------------------------ near-symmetric records
+// One wave per element of a one-element FP64 tile group stored as
+// near-symmetric records (amx_nsym.hpp).  The wave copies the record's U and
+// X (16-byte loads, every lane) and its gathered u_e into its LDS region; lane
+// r loads its delta chunks into registers.  Lane r then forms row r with c
+// ascending from U[c][r] plus its delta (c < r: consecutive lanes read
+// consecutive LDS words) or U[r][c] (c >= r): the products and sums of the
+// whole layout in the same order, so the result is bit-identical.  The LDS
+// word of lane r moves by n - c - 1 values from column c to c + 1 while c < r
+// and by one after that, so no address table is needed; columns that hold an
+// exception in some row (colany, the same for every element of the group)
+// also move the lane's cursor over its X values.
+template <int N> constexpr uint32_t nsym_rounds() { return (nsym_nsp(N) / 2u + 63u) / 64u + 2u; }
+// exception values a record may hold: what the copy rounds hold beyond U
+template <int N> constexpr uint32_t nsym_max_x() { return nsym_rounds<N>() * 128u - nsym_nsp(N); }
+
+// LDS values of one wave's region: U, X and u_e
+__host__ __device__ constexpr uint32_t nsym_region_values(uint32_t n, uint32_t nx)
+{
+    return nsym_nsp(n) + nsym_nxp(nx) + 64u;
+}
+
+// f(integral_constant<int, I>) for I = B .. E-1
+template <int B, int E, typename F> __device__ __forceinline__ void static_for(F&& f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+template <int N, bool MASKED>
+__device__ __forceinline__ double ebe_nsym_acc(const uint8_t* __restrict__ rec, uint32_t nxp, uint32_t xs,
+                                               uint64_t colany, const uint32_t* __restrict__ map,
+                                               const double* __restrict__ u, uint64_t slot, double* wl, uint32_t& m)
+{
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    constexpr uint32_t NSP = nsym_nsp(N), NR = nsym_rounds<N>(), NQ = nsym_nq(N);
+    const uint32_t lane = threadIdx.x & 63u;
+    // lanes past N mirror the last row (same addresses, no traffic)
+    const uint32_t lc = lane < (uint32_t)N ? lane : (uint32_t)N - 1u;
+    const uint32_t nch = (NSP + nxp) / 2u;  // 16-byte chunks of U and X
+    // the map entry first (the u gather waits for it, not for the record);
+    // every load unconditional (indices clamped into the record), so that no
+    // wait lands between them
+    m = map[slot * N + lc];
+    __builtin_amdgcn_sched_barrier(0);
+    const u32x4* src = reinterpret_cast<const u32x4*>(rec);
+    u32x4 ch[NR];
+#pragma unroll
+    for (uint32_t i = 0; i < NR; i++) ch[i] = __builtin_nontemporal_load(src + min(lane + 64u * i, nch - 1u));
+    const u32x4* dsrc = src + nch;
+    u32x4 dq[NQ];
+#pragma unroll
+    for (uint32_t q = 0; q < NQ; q++)
+        dq[q] = __builtin_nontemporal_load(dsrc + nsym_dbase(N, q) + (lc >= 16u * q + 1u ? lc - 16u * q - 1u : 0u));
+    // keep the record's loads ahead of the u gather (and unsunk): they are
+    // the long stream, the gather waits only for the map entry
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    double uv = u[m & kIdxMask];
+    if (MASKED && (m & kFlagBit)) uv = 0.0;
+    // rows r <= 16q have no lower entry in chunk column q
+#pragma unroll
+    for (uint32_t q = 0; q < NQ; q++)
+        if (lc < 16u * q + 1u) dq[q] = u32x4{0u, 0u, 0u, 0u};
+    u32x4* dst = reinterpret_cast<u32x4*>(wl);
+#pragma unroll
+    for (uint32_t i = 0; i < NR; i++)
+        if (lane + 64u * i < nch) dst[lane + 64u * i] = ch[i];
+    double* ush = wl + NSP + nxp;
+    ush[lane] = uv;
+    wave_lds_sync();
+    // LDS word indices in the region: the lane's word of column c, its next X value
+    const uint64_t* kw = reinterpret_cast<const uint64_t*>(wl);
+    uint32_t pa = lc, px = NSP + xs;
+    double acc = 0.0;
+    static_for<0, N>([&](auto ci) {
+        constexpr int c = decltype(ci)::value;
+        const uint32_t wd = dq[c / 16][(c % 16) / 4];
+        const int32_t d = ((int32_t)(wd << (24 - 8 * (c % 4)))) >> 24;
+        // some row has an exception in column c (uniform) and this lane's is one
+        const bool exc = ((colany >> c) & 1u) != 0 && d == -128;
+        uint64_t bits = kw[exc ? px : pa];
+        px += exc ? 1u : 0u;
+        bits = (bits & 0xffffffff00000000ull) | (uint32_t)((uint32_t)bits + (uint32_t)d);
+        acc = madd(acc, __longlong_as_double((long long)bits), ush[c]);
+        // lanes r > c (r = lc) step along row c of U, the others along their own row
+        pa += (int)lane > c ? (uint32_t)(N - c - 1) : 1u;
+    });
+    return acc;
+}
+
+// one element per wave, blockDim.x / 64 waves per block, dynamic LDS of
+// nsym_region_values(N, nx) doubles per wave
+template <int N, bool MASKED, bool ATOMIC>
+__global__ __launch_bounds__(256) void ebe_nsym_kernel(const uint8_t* __restrict__ K, uint64_t rec_bytes,
+                                                      uint32_t nxp, const uint32_t* __restrict__ xs, uint64_t colany,
+                                                      const uint32_t* __restrict__ map, const double* __restrict__ u,
+                                                      double* __restrict__ out, uint64_t ve_base, uint32_t rs,
+                                                      uint32_t ss, uint32_t tile_begin, uint32_t n_tiles,
+                                                      const uint32_t* __restrict__ skip)
+{
+    extern __shared__ __align__(16) double nsym_lds[];
+    if (skip && *skip) return;  // converged solver: nothing to do
+    const uint32_t wpb = blockDim.x >> 6, w = threadIdx.x >> 6;
+    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * wpb + w;
+    if (t >= n_tiles) return;  // uniform per wave
+    const uint64_t slot = (uint64_t)tile_begin + t;
+    const uint32_t lane = threadIdx.x & 63u, lc = lane < (uint32_t)N ? lane : (uint32_t)N - 1u;
+    uint32_t m;
+    const double acc = ebe_nsym_acc<N, MASKED>(K + slot * rec_bytes, nxp, xs[lc], colany, map, u, slot,
+                                               nsym_lds + w * (nsym_nsp(N) + nxp + 64u), m);
+    if (lane < (uint32_t)N) {
+        if constexpr (ATOMIC) atomic_add(&out[m & kIdxMask], acc);
+        else out[ve_base + (uint64_t)lane * rs + slot * ss] = acc;
+    }
+}
+
 // index of the entry that owns work item x: the largest j < n with prefix[j] <= x
 // (x wave-uniform, so the search runs on the scalar unit)
 __device__ __forceinline__ uint32_t owner_of(const uint32_t* __restrict__ prefix, uint32_t n, uint32_t x)
@@ -443,6 +563,21 @@ __global__ __launch_bounds__(256) void ebe_tile_batched_kernel(const EbeJob* __r
     const EbeJob& jb = jobs[j];
     if (jb.skip && *jb.skip) return;
     const uint32_t t = jb.tile_begin + (gt - prefix[j]);
+    if constexpr (N == 60 && std::is_same<KT, double>::value && std::is_same<VT, double>::value) {
+        if (jb.nsym) {  // near-symmetric records: one region per wave in the launch's dynamic LDS
+            extern __shared__ __align__(16) double nsym_lds[];
+            const uint32_t nxp = nsym_nxp(jb.nsym_nx), lane = threadIdx.x & 63u, lc = lane < 60u ? lane : 59u;
+            const uint8_t* rec = static_cast<const uint8_t*>(jb.K) + (uint64_t)t * nsym_record_bytes_of(60, jb.nsym_nx);
+            double* wl = nsym_lds + (threadIdx.x >> 6) * nsym_region_values(60, nsym_max_x<60>());
+            uint32_t m;
+            const double acc = jb.masked ? ebe_nsym_acc<60, true>(rec, nxp, jb.nsym_xs[lc], jb.nsym_colany, jb.map,
+                                                                  static_cast<const double*>(jb.u), t, wl, m)
+                                         : ebe_nsym_acc<60, false>(rec, nxp, jb.nsym_xs[lc], jb.nsym_colany, jb.map,
+                                                                   static_cast<const double*>(jb.u), t, wl, m);
+            if (lane < 60u) static_cast<double*>(jb.out)[jb.ve_base + (uint64_t)lane * jb.ve_rs + (uint64_t)t * jb.ve_ss] = acc;
+            return;
+        }
+    }
     if constexpr (SYM_OK) {
         if (jb.sym) {
             KT* reg = ksh + (threadIdx.x >> 6) * sym_region_values<N, KT>();
@@ -659,6 +794,118 @@ __global__ void relayout_sym_kernel(const double* __restrict__ src, const double
     dst[idx] = from_f64<KT>(val);
 }
 
+// ---- near-symmetric records (amx_nsym.hpp)
+__device__ __forceinline__ double src_value(const double* __restrict__ src, const double* __restrict__ kref,
+                                            const double* __restrict__ scale, uint32_t e, uint32_t N, uint32_t r,
+                                            uint32_t c)
+{
+    return src ? src[(uint64_t)e * N * N + (uint64_t)r * N + c] : scale[e] * kref[(uint64_t)r * N + c];
+}
+
+// lower entry k of a row-major strict lower triangle -> (r, c), r > c
+__device__ __forceinline__ void lower_rc(uint32_t k, uint32_t& r, uint32_t& c)
+{
+    r = (uint32_t)((1.0 + sqrt(1.0 + 8.0 * (double)k)) * 0.5);
+    while (r * (r - 1u) / 2u > k) r--;
+    while ((r + 1u) * r / 2u <= k) r++;
+    c = k - r * (r - 1u) / 2u;
+}
+
+// K[r][c] coded from K[c][r] by a byte on the low word (same high word,
+// difference in [-127, 127])
+__device__ __forceinline__ bool nsym_codable(double lower, double upper, int32_t& d)
+{
+    const uint64_t a = (uint64_t)__double_as_longlong(lower), b = (uint64_t)__double_as_longlong(upper);
+    const int64_t diff = (int64_t)(a & 0xffffffffull) - (int64_t)(b & 0xffffffffull);
+    d = (int32_t)diff;
+    return (a >> 32) == (b >> 32) && diff >= -127 && diff <= 127;
+}
+
+// flags[r*N + c] = 1 for the lower entries that some matrix of the group
+// cannot code (one thread per slot and lower entry; every writer stores 1)
+__global__ void nsym_scan_kernel(const double* __restrict__ src, const double* __restrict__ kref,
+                                 const double* __restrict__ scale, const uint32_t* __restrict__ slot_elem,
+                                 uint64_t total, uint32_t N, uint32_t* __restrict__ flags)
+{
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const uint32_t NL = N * (N - 1u) / 2u;
+    const uint64_t slot = idx / NL;
+    const uint32_t e = slot_elem[slot];
+    if (e == kPadSlot) return;
+    uint32_t r, c;
+    lower_rc((uint32_t)(idx - slot * NL), r, c);
+    int32_t d;
+    if (!nsym_codable(src_value(src, kref, scale, e, N, r, c), src_value(src, kref, scale, e, N, c, r), d))
+        flags[r * N + c] = 1u;
+}
+
+// records, one thread per 32-bit word.  xrc[x]: r*N + c of exception x;
+// xidx[r*N + c]: its index, 0xffff elsewhere.
+__global__ void relayout_nsym_kernel(const double* __restrict__ src, const double* __restrict__ kref,
+                                     const double* __restrict__ scale, const uint32_t* __restrict__ slot_elem,
+                                     uint64_t total, uint32_t N, uint32_t nx, const uint16_t* __restrict__ xrc,
+                                     const uint16_t* __restrict__ xidx, uint32_t* __restrict__ dst)
+{
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const uint32_t RW = (uint32_t)(nsym_record_bytes_of(N, nx) / 4u);
+    const uint64_t slot = idx / RW;
+    const uint32_t w = (uint32_t)(idx - slot * RW);
+    const uint32_t e = slot_elem[slot];
+    const uint32_t NSP = nsym_nsp(N), UXW = 2u * (NSP + nsym_nxp(nx));
+    uint32_t out = 0u;
+    if (e != kPadSlot) {
+        if (w < UXW) {  // U and X values, two words each
+            const uint32_t v = w / 2u;
+            uint64_t bits = 0;
+            if (v < nsym_ns(N)) {
+                uint32_t a = 0, k = v;
+                while (k >= N - a) {  // row a of the upper triangle holds N - a values
+                    k -= N - a;
+                    a++;
+                }
+                bits = (uint64_t)__double_as_longlong(src_value(src, kref, scale, e, N, a, a + k));
+            } else if (v >= NSP && v < NSP + nx) {
+                const uint32_t rc = xrc[v - NSP];
+                bits = (uint64_t)__double_as_longlong(src_value(src, kref, scale, e, N, rc / N, rc % N));
+                bits = (bits & 0xffffffff00000000ull) | (uint32_t)((uint32_t)bits + 128u);
+            }
+            out = (w & 1u) ? (uint32_t)(bits >> 32) : (uint32_t)bits;
+        } else {  // delta bytes, four per word
+            const uint32_t chunk = (w - UXW) / 4u, b0 = ((w - UXW) % 4u) * 4u;
+            uint32_t q = 0;
+            while (chunk >= nsym_dbase(N, q + 1u)) q++;
+            const uint32_t r = chunk - nsym_dbase(N, q) + 16u * q + 1u;
+            for (uint32_t b = 0; b < 4u; b++) {
+                const uint32_t c = 16u * q + b0 + b;
+                int32_t d = 0;
+                if (c < r) {
+                    if (xidx[r * N + c] != 0xffffu) {
+                        d = -128;
+                    } else {
+                        (void)nsym_codable(src_value(src, kref, scale, e, N, r, c),
+                                           src_value(src, kref, scale, e, N, c, r), d);
+                    }
+                }
+                out |= ((uint32_t)d & 0xffu) << (8u * b);
+            }
+        }
+    }
+    dst[idx] = out;
+}
+
+// diagonal entries of near-symmetric records into ve (row (slot, r) at ve[r*rs + slot*ss])
+__global__ void nsym_diag_kernel(const double* __restrict__ K, uint32_t N, uint64_t rec_values, uint64_t n_rows,
+                                 uint32_t rs, uint32_t ss, double* __restrict__ ve)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;  // slot * N + r
+    if (i >= n_rows) return;
+    const uint64_t slot = i / N;
+    const uint32_t r = (uint32_t)(i - slot * N);
+    ve[(uint64_t)r * rs + slot * ss] = K[slot * rec_values + nsym_uidx(N, r, r)];
+}
+
 // FP32 diagonal of the BF16 storage: d_r = K_rr + sum_{j != r} (K_rj - bf16(K_rj)),
 // so that every row of the stored matrix sums to the FP64 row sum (constants
 // stay in the null space of a Laplacian element); symmetric K stays symmetric.
@@ -785,6 +1032,32 @@ void launch_tile(bool masked, bool atomic, bool sym, const EbeArgs& a, hipStream
 #undef AMX_TILE
 }
 
+// waves per block of the near-symmetric kernel
+constexpr uint32_t kNsymWaves = 1;
+
+template <int N>
+void launch_nsym_n(const Layout& L, bool masked, bool atomic, const EbeArgs& a, hipStream_t s)
+{
+    const uint32_t nxp = nsym_nxp(L.nx);
+    const dim3 grid(blocks_for(a.n_tiles, kNsymWaves)), block(64 * kNsymWaves);
+    const size_t lds = (size_t)kNsymWaves * nsym_region_values(N, L.nx) * sizeof(double);
+    const uint8_t* K = static_cast<const uint8_t*>(a.K);
+    const uint64_t rb = nsym_record_bytes_of(N, L.nx);
+    const double* u = static_cast<const double*>(a.u);
+    double* out = static_cast<double*>(a.out);
+#define AMX_NSYM(M, A)                                                                                             \
+    ebe_nsym_kernel<N, M, A><<<grid, block, lds, s>>>(K, rb, nxp, a.nsym_xs, a.nsym_colany, a.map, u, out,        \
+                                                      a.ve_base, a.ve_rs, a.ve_ss, a.tile_begin, a.n_tiles, a.skip)
+    if (masked) {
+        if (atomic) AMX_NSYM(true, true);
+        else AMX_NSYM(true, false);
+    } else {
+        if (atomic) AMX_NSYM(false, true);
+        else AMX_NSYM(false, false);
+    }
+#undef AMX_NSYM
+}
+
 template <typename KT, typename VT>
 void launch_generic(uint32_t n, bool masked, bool atomic, const EbeArgs& a, hipStream_t s)
 {
@@ -833,8 +1106,19 @@ size_t ktype_size(KType kt) { return kt == KType::F64 ? 8 : kt == KType::F32 ? 4
 
 bool sym_supported(uint32_t n) { return n == 8 || n == 10; }
 
+bool nsym_supported(uint32_t n) { return n == 60; }
+
+uint32_t nsym_max_exceptions(uint32_t n)
+{
+    AMX_REQUIRE(nsym_supported(n), AMX_INVALID_ARGUMENT, "no near-symmetric kernel for n = " + std::to_string(n));
+    return nsym_max_x<60>();
+}
+
+uint64_t nsym_record_bytes(uint32_t n, uint32_t nx) { return nsym_record_bytes_of(n, nx); }
+
 uint64_t tile_values(const Layout& L, KType kt)
 {
+    if (L.nsym) return nsym_record_bytes_of(L.n, L.nx) / ktype_size(kt);
     if (L.sym) return sym_tile_values(L.n, (uint32_t)ktype_size(kt));
     return (uint64_t)L.G * L.n * L.n;
 }
@@ -913,11 +1197,37 @@ void launch_bf16_diag(const double* src_rowmajor, const double* k_ref, const dou
     AMX_HIP(hipGetLastError());
 }
 
+void launch_nsym_scan(const double* src, const double* k_ref, const double* scale, const uint32_t* slot_elem,
+                      uint32_t n_slots, uint32_t n, uint32_t* flags, hipStream_t s)
+{
+    const uint64_t total = (uint64_t)n_slots * (n * (n - 1u) / 2u);
+    if (total == 0) return;
+    nsym_scan_kernel<<<blocks_for(total, 256), 256, 0, s>>>(src, k_ref, scale, slot_elem, total, n, flags);
+    AMX_HIP(hipGetLastError());
+}
+
+void launch_relayout_nsym(const double* src, const double* k_ref, const double* scale, const uint32_t* slot_elem,
+                          uint32_t n_slots, const Layout& L, const uint16_t* xrc, const uint16_t* xidx, void* dst,
+                          hipStream_t s)
+{
+    const uint64_t total = (uint64_t)n_slots * (nsym_record_bytes_of(L.n, L.nx) / 4u);
+    if (total == 0) return;
+    relayout_nsym_kernel<<<blocks_for(total, 256), 256, 0, s>>>(src, k_ref, scale, slot_elem, total, L.n, L.nx, xrc,
+                                                                xidx, static_cast<uint32_t*>(dst));
+    AMX_HIP(hipGetLastError());
+}
+
 void launch_extract_diag(const double* K, const Layout& L, uint32_t n_slots, uint32_t rs, uint32_t ss, double* ve,
                          hipStream_t s)
 {
     const uint64_t rows = (uint64_t)n_slots * L.n;
     if (rows == 0) return;
+    if (L.nsym) {
+        nsym_diag_kernel<<<blocks_for(rows, 256), 256, 0, s>>>(K, L.n, nsym_record_bytes_of(L.n, L.nx) / 8u, rows,
+                                                               rs, ss, ve);
+        AMX_HIP(hipGetLastError());
+        return;
+    }
     const uint32_t te = L.sym ? (uint32_t)tile_values(L, KType::F64) : 0u;
     extract_diag_kernel<<<blocks_for(rows, 256), 256, 0, s>>>(K, TiledIndex{L.n, L.G, L.P}, te, rows, rs, ss, ve);
     AMX_HIP(hipGetLastError());
@@ -926,6 +1236,13 @@ void launch_extract_diag(const double* K, const Layout& L, uint32_t n_slots, uin
 void launch_ebe(const Layout& L, KType kt, bool vt_double, bool masked, bool atomic, const EbeArgs& a, hipStream_t s)
 {
     if (a.n_tiles == 0) return;
+    if (L.nsym) {
+        AMX_REQUIRE(kt == KType::F64 && vt_double && L.n == 60, AMX_INVALID_STATE,
+                    "near-symmetric records hold FP64 matrices of 60 DoFs");
+        launch_nsym_n<60>(L, masked, atomic, a, s);
+        AMX_HIP(hipGetLastError());
+        return;
+    }
     if (vt_double) {
         // FP64 vectors: FP64 storage, or FP32 / BF16 storage widened on load
         if (kt == KType::F64) dispatch_n<double, double>(L, masked, atomic, a, s);
@@ -962,11 +1279,20 @@ void launch_gather(bool vt_double, int bc, const uint32_t* off, const uint32_t* 
 }
 
 void launch_ebe_batched(uint32_t n, KType kt, bool vt_double, const EbeJob* jobs, const uint32_t* prefix,
-                        uint32_t n_jobs, uint32_t total_tiles, hipStream_t s)
+                        uint32_t n_jobs, uint32_t total_tiles, hipStream_t s, bool any_nsym)
 {
     if (total_tiles == 0 || n_jobs == 0) return;
     const uint32_t nb = blocks_for(total_tiles, 4);
-#define AMX_EBE_B(NN, KTT, VTT) ebe_tile_batched_kernel<NN, KTT, VTT><<<nb, 256, 0, s>>>(jobs, prefix, n_jobs)
+    // near-symmetric records: four regions of the largest record a job may have
+    size_t lds = 0;
+    if (any_nsym) {
+        AMX_REQUIRE(n == 60 && kt == KType::F64 && vt_double, AMX_INVALID_STATE,
+                    "near-symmetric records hold FP64 matrices of 60 DoFs");
+        lds = 4u * nsym_region_values(60, nsym_max_x<60>()) * sizeof(double);
+        AMX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&ebe_tile_batched_kernel<60, double, double>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+#define AMX_EBE_B(NN, KTT, VTT) ebe_tile_batched_kernel<NN, KTT, VTT><<<nb, 256, lds, s>>>(jobs, prefix, n_jobs)
 #define AMX_EBE_N(KTT, VTT)                                                                                        \
     switch (n) {                                                                                                   \
     case 4: AMX_EBE_B(4, KTT, VTT); break;                                                                         \

@@ -23,16 +23,30 @@ enum class KType : int { F64 = 0, F32 = 1, BF16 = 2 };
 // a*n - a(a-1)/2 + (b - a) (row-major upper triangle); TE = G*NS rounded up
 // to 16 bytes.  The kernels read K[r][c] = K[min][max]: the same values in the
 // same order as the full layout, about half the bytes.
+//
+// Near-symmetric records (nsym, one-element tiles of FP64 storage, n = 60):
+// when every matrix of the group is symmetric up to small differences in the
+// low word of the transposed entries (quadrature round-off) and at most
+// nsym_max_exceptions(n) lower positions need their own values, each element
+// is one record of nsym_record_bytes(n, nx) bytes: upper triangle, the nx
+// exception values and one byte per lower entry (amx_nsym.hpp).  The kernel
+// rebuilds every K[r][c] bit for bit and keeps the whole layout's order.
 struct Layout {
     uint32_t n = 0, G = 1, P = 1;
     bool specialized = false;  // a template kernel exists for (n, G, P, type)
     bool sym = false;          // packed symmetric tiles
+    bool nsym = false;         // near-symmetric records
+    uint32_t nx = 0;           // nsym: exception values per record
 };
 
 Layout choose_layout(uint32_t n, KType kt);
 size_t ktype_size(KType kt);
 // packed symmetric tiles exist for these DoF counts (multi-element tiles)
 bool sym_supported(uint32_t n);
+// near-symmetric records exist for these DoF counts (one-element FP64 tiles)
+bool nsym_supported(uint32_t n);
+uint32_t nsym_max_exceptions(uint32_t n);
+uint64_t nsym_record_bytes(uint32_t n, uint32_t nx);
 // values per stored tile of layout L in storage kt (padding included)
 uint64_t tile_values(const Layout& L, KType kt);
 
@@ -48,6 +62,16 @@ void launch_relayout_scaled(const double* k_ref, const double* scale, const uint
 // row-major matrices (src_rowmajor) or k_ref + scale.
 void launch_bf16_diag(const double* src_rowmajor, const double* k_ref, const double* scale, const uint32_t* slot_elem,
                       uint32_t n_slots, uint32_t n, float* out, hipStream_t s);
+// near-symmetric records: flags[r*n + c] (r > c) set to 1 where some matrix of
+// the group cannot code K[r][c] from K[c][r] (flags zeroed by the caller);
+// source either row-major matrices (src) or k_ref + scale
+void launch_nsym_scan(const double* src, const double* k_ref, const double* scale, const uint32_t* slot_elem,
+                      uint32_t n_slots, uint32_t n, uint32_t* flags, hipStream_t s);
+// the records of layout L (L.nsym, L.nx): xrc[x] = r*n + c of exception x in
+// row-major order, xidx[r*n + c] = x (0xffff elsewhere)
+void launch_relayout_nsym(const double* src, const double* k_ref, const double* scale, const uint32_t* slot_elem,
+                          uint32_t n_slots, const Layout& L, const uint16_t* xrc, const uint16_t* xidx, void* dst,
+                          hipStream_t s);
 // diagonal entries K_e(r,r) of every slot row from the tiled FP64 storage into ve
 // (row (slot, r) at ve[r*rs + slot*ss]).
 void launch_extract_diag(const double* K, const Layout& L, uint32_t n_slots, uint32_t rs, uint32_t ss, double* ve,
@@ -64,6 +88,8 @@ struct EbeArgs {
     uint32_t tile_begin, n_tiles;
     const uint32_t* skip = nullptr;  // device flag: nonzero -> kernel exits (converged solver)
     const float* kdiag = nullptr;    // BF16 storage: FP32 diagonal [slot][r]
+    const uint32_t* nsym_xs = nullptr;  // near-symmetric records: first X index of each row
+    uint64_t nsym_colany = 0;           // bit c: some row has an exception in column c
 };
 // vt_double: vectors/accumulation in FP64 (only with KType::F64), else FP32.
 void launch_ebe(const Layout& L, KType kt, bool vt_double, bool masked, bool atomic, const EbeArgs& a,
@@ -90,6 +116,9 @@ struct EbeJob {
     const uint32_t* skip;   // the job's converged flag
     const float* kdiag;
     uint32_t masked, sym;   // sym: packed symmetric tiles
+    const uint32_t* nsym_xs;  // near-symmetric records (EbeArgs), nsym_nx exceptions
+    uint64_t nsym_colany;
+    uint32_t nsym, nsym_nx;
 };
 struct GatherJob {
     const uint32_t* off;
@@ -103,8 +132,10 @@ struct GatherJob {
     const uint32_t* skip;
 };
 bool batched_n(uint32_t n);
+// any_nsym: some job has near-symmetric records (the launch then carries
+// their LDS regions)
 void launch_ebe_batched(uint32_t n, KType kt, bool vt_double, const EbeJob* jobs, const uint32_t* prefix,
-                        uint32_t n_jobs, uint32_t total_tiles, hipStream_t s);
+                        uint32_t n_jobs, uint32_t total_tiles, hipStream_t s, bool any_nsym = false);
 void launch_gather_batched(bool vt_double, const GatherJob* jobs, const uint32_t* prefix, uint32_t n_jobs,
                            uint32_t total_blocks, hipStream_t s);
 

@@ -26,6 +26,7 @@
 #include <cstring>
 #include <vector>
 
+#include "amx_nsym.hpp"
 #include "amx_op.hpp"
 
 using amx::DevBuf;
@@ -578,9 +579,13 @@ __global__ __launch_bounds__(256) void mrhs_gather4_kernel(const uint32_t* __res
 // element matrices for the MFMA kernels from the tiled FP64 storage:
 // 60 DoFs -> compact MFMA order (pos60); smaller -> FP32 column-major n x n or
 // BF16 row-major n x ld.  One thread per stored value.
+// K64: the operator's FP64 storage -- column-packed tiles (G, P), packed
+// symmetric tiles (sym_te values per tile) or near-symmetric records (nsym_rec
+// values per record, nsym_nx exceptions, nsym_xidx)
 __global__ void mrhs_layout_kernel(const double* __restrict__ K64, uint32_t N, uint32_t G, uint32_t P,
-                                   uint32_t sym_te, const uint32_t* __restrict__ slot_of, uint32_t n_elem, int bf16,
-                                   uint32_t ld, void* __restrict__ out)
+                                   uint32_t sym_te, uint64_t nsym_rec, uint32_t nsym_nx,
+                                   const uint16_t* __restrict__ nsym_xidx, const uint32_t* __restrict__ slot_of,
+                                   uint32_t n_elem, int bf16, uint32_t ld, void* __restrict__ out)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t per = (bf16 && N != 60) ? (uint64_t)N * ld : (uint64_t)N * N;
@@ -605,7 +610,9 @@ __global__ void mrhs_layout_kernel(const double* __restrict__ K64, uint32_t N, u
         const uint64_t slot = slot_of[e];
         const uint64_t t = slot / G;
         const uint32_t g = (uint32_t)(slot - t * G), GN = G * N, NP = N / P;
-        if (sym_te) {  // packed symmetric tiles: K[r][c] = K[min][max] of the upper triangle
+        if (nsym_rec) {
+            val = (float)nsym_value(K64 + slot * nsym_rec, N, nsym_nx, nsym_xidx, r, c);
+        } else if (sym_te) {  // packed symmetric tiles: K[r][c] = K[min][max] of the upper triangle
             const uint32_t a = r < c ? r : c, b = r < c ? c : r;
             val = (float)K64[t * sym_te + g * (N * (N + 1) / 2) + a * N - (a * (a - 1)) / 2 + (b - a)];
         } else {
@@ -829,7 +836,9 @@ int amx_mrhs_create(amx_op* const* ops, int n_ops, int precision, amx_mrhs** out
             if (total)
                 amx::mrhs_layout_kernel<<<(uint32_t)((total + 255) / 256), 256, 0, op.stream>>>(
                     g.K[0].as<double>(), g.n, g.L[0].G, g.L[0].P,
-                    g.L[0].sym ? (uint32_t)amx::tile_values(g.L[0], amx::KType::F64) : 0u, dslot.as<uint32_t>(),
+                    g.L[0].sym ? (uint32_t)amx::tile_values(g.L[0], amx::KType::F64) : 0u,
+                    g.L[0].nsym ? amx::nsym_record_bytes(g.n, g.L[0].nx) / 8u : 0u, g.L[0].nx,
+                    g.nsym_xidx.as<uint16_t>(), dslot.as<uint32_t>(),
                     p.n_elem, m->bf16 ? 1 : 0, ld,
                     p.K.p);
             AMX_HIP(hipGetLastError());

@@ -176,6 +176,9 @@ void finalize_op(Op& op)
         g.xe.release();
         g.xe32.release();
         g.fscale.release();
+        g.nsym_xs.release();
+        g.nsym_xidx.release();
+        g.nsym_colany = 0;
     }
     const bool free_m = op.is_free();
     if (free_m) {
@@ -277,13 +280,53 @@ void finalize_op(Op& op)
             uint64_t tmp = 0;
             upload(dslot, slot_src, s, tmp);
         }
+        const double* src_rm = op.scaled ? nullptr : src.as<double>();
+        const double* src_kref = op.scaled ? src.as<double>() : nullptr;
+        const double* src_sc = op.scaled ? src_scale.as<double>() : nullptr;
+        // near-symmetric records for one-element FP64 tiles: find the lower
+        // positions that some matrix cannot code from its transpose; few
+        // enough of them -> records, else the whole layout
+        DevBuf dxrc;
+        if (op.sym_mode == 1 && g.G == 1 && nsym_supported(n) && g.n_slots > 0) {
+            DevBuf flags;
+            flags.alloc((size_t)n * n * sizeof(uint32_t));
+            AMX_HIP(hipMemsetAsync(flags.p, 0, flags.bytes, s));
+            launch_nsym_scan(src_rm, src_kref, src_sc, dslot.as<uint32_t>(), g.n_slots, n, flags.as<uint32_t>(), s);
+            std::vector<uint32_t> hf((size_t)n * n);
+            AMX_HIP(hipMemcpyAsync(hf.data(), flags.p, flags.bytes, hipMemcpyDeviceToHost, s));
+            AMX_HIP(hipStreamSynchronize(s));
+            std::vector<uint16_t> xrc, xidx((size_t)n * n, 0xffffu);
+            std::vector<uint32_t> xs(n, 0);
+            uint64_t colany = 0;
+            for (uint32_t r = 0; r < n; r++) {
+                xs[r] = (uint32_t)xrc.size();
+                for (uint32_t c = 0; c < r; c++) {
+                    if (!hf[(size_t)r * n + c]) continue;
+                    xidx[(size_t)r * n + c] = (uint16_t)xrc.size();
+                    xrc.push_back((uint16_t)(r * n + c));
+                    colany |= 1ull << c;
+                }
+            }
+            if (xrc.size() <= nsym_max_exceptions(n)) {
+                g.L[0].nsym = true;
+                g.L[0].nx = (uint32_t)xrc.size();
+                g.nsym_colany = colany;
+                upload(g.nsym_xs, xs, s, op.device_bytes);
+                upload(g.nsym_xidx, xidx, s, op.device_bytes);
+                uint64_t tmp = 0;
+                upload(dxrc, xrc, s, tmp);
+            }
+        }
         const int bits[3] = {AMX_PREC_FP64, AMX_PREC_FP32, AMX_PREC_BF16};
         for (int kt = 0; kt < 3; kt++) {
             if (kt > 0 && !op.has(bits[kt])) continue;
             const uint64_t bytes = (uint64_t)g.n_tiles * tile_values(g.L[kt], (KType)kt) * ktype_size((KType)kt);
             g.K[kt].alloc(std::max<uint64_t>(bytes, 16));
             op.device_bytes += g.K[kt].bytes;
-            if (op.scaled)
+            if (kt == 0 && g.L[0].nsym)
+                launch_relayout_nsym(src_rm, src_kref, src_sc, dslot.as<uint32_t>(), g.n_slots, g.L[0],
+                                     dxrc.as<uint16_t>(), g.nsym_xidx.as<uint16_t>(), g.K[0].p, s);
+            else if (op.scaled)
                 launch_relayout_scaled(src.as<double>(), src_scale.as<double>(), dslot.as<uint32_t>(), g.n_slots,
                                        g.L[kt], (KType)kt, g.K[kt].p, s);
             else
@@ -522,6 +565,8 @@ void ebe_launch(Op& op, KType kt, bool vt_double, bool masked, int phase, const 
         a.ve_ss = g.ve_ss;
         a.skip = skip;
         a.kdiag = kt == KType::BF16 ? g.kdiag16.as<float>() : nullptr;
+        a.nsym_xs = g.nsym_xs.as<uint32_t>();
+        a.nsym_colany = g.nsym_colany;
         if (phase == 0) { a.tile_begin = 0; a.n_tiles = g.tiles_phase0; }
         else if (phase == 1) { a.tile_begin = g.tiles_phase0; a.n_tiles = g.n_tiles - g.tiles_phase0; }
         else { a.tile_begin = 0; a.n_tiles = g.n_tiles; }
@@ -842,13 +887,16 @@ int amx_op_get_info(const amx_op* h, amx_op_info* info)
                 info->flops += ne * amx::free_flops_per_elem(op.elem_kind, op.elem_nq);
                 continue;
             }
-            // values stored per element: the upper triangle of packed symmetric tiles
+            // values stored per element: the upper triangle of packed symmetric
+            // tiles; near-symmetric records (FP64 only) by their record size
             const uint64_t ns = g.L[0].sym ? (uint64_t)g.n * (g.n + 1) / 2 : nn;
+            const uint64_t b64 = g.L[0].nsym ? amx::nsym_record_bytes(g.n, g.L[0].nx) : ns * 8;
             if (g.L[0].sym) info->packed_symmetric++;
-            info->k_bytes_fp64 += ne * ns * 8;
+            if (g.L[0].nsym) info->near_symmetric++;
+            info->k_bytes_fp64 += ne * b64;
             info->k_bytes_fp32 += ne * ns * 4;
             info->k_bytes_bf16 += ne * ns * 2;
-            info->alg_bytes_fp64 += ne * (ns * 8 + 4ull * g.n);
+            info->alg_bytes_fp64 += ne * (b64 + 4ull * g.n);
             info->flops += 2 * ne * nn;
         }
         info->alg_bytes_fp64 += 16ull * op.n_total;

@@ -33,6 +33,10 @@ struct Group {
     DevBuf map;                       // [slot][r] local ids | identity-BC flag
     DevBuf kdiag16;                   // BF16 storage: FP32 corrected diagonal [slot][r]
     DevBuf xe, xe32, fscale;          // FREE: coordinates (FP64 and an FP32 copy) and factors per slot
+    // near-symmetric records (L[0].nsym): first exception index of each row,
+    // exception index of each (r, c) (0xffff: none), columns holding one
+    DevBuf nsym_xs, nsym_xidx;
+    uint64_t nsym_colany = 0;
 };
 
 struct Op {
@@ -44,7 +48,8 @@ struct Op {
     int bc_operator = AMX_BCOP_IDENTITY;
     int scatter = AMX_SCATTER_DETERMINISTIC;
     int prec_mask = AMX_PREC_FP64;
-    int sym_mode = 1;                    // packed symmetric tiles: 0 never, 1 when every matrix is symmetric
+    int sym_mode = 1;                    // packed symmetric tiles / near-symmetric records: 0 never, 1 when
+                                         // the group's matrices allow them
     bool penalty_set = false;
     double penalty_value = 0.0;
     double trace = 0.0;

@@ -74,6 +74,7 @@ struct Job {
 struct Table {
     DevBuf dev_jobs, dev_prefix;
     uint32_t n = 0, total = 0;
+    bool any_nsym = false;  // element tables: some job has near-symmetric records
 };
 
 }  // namespace
@@ -184,6 +185,10 @@ void rebuild_tables(amx_server& S)
             a.kdiag = j.kt == KType::BF16 ? grp.kdiag16.as<float>() : nullptr;
             a.masked = op.bc_operator == AMX_BCOP_IDENTITY ? 1u : 0u;
             a.sym = grp.L[(int)j.kt].sym ? 1u : 0u;
+            a.nsym = grp.L[(int)j.kt].nsym ? 1u : 0u;
+            a.nsym_nx = grp.L[(int)j.kt].nx;
+            a.nsym_xs = grp.nsym_xs.as<uint32_t>();
+            a.nsym_colany = grp.nsym_colany;
             e.first.push_back(a);
             e.second.push_back(e.second.back() + grp.n_tiles);
         }
@@ -217,7 +222,12 @@ void rebuild_tables(amx_server& S)
         cp[f].push_back(cp[f].back() + cj.nblk);
     }
     S.ebe.clear();
-    for (auto& kv : ebe) upload_table(S.ebe[kv.first], kv.second.first, kv.second.second);
+    for (auto& kv : ebe) {
+        Table& t = S.ebe[kv.first];
+        upload_table(t, kv.second.first, kv.second.second);
+        t.any_nsym = std::any_of(kv.second.first.begin(), kv.second.first.end(),
+                                 [](const amx::EbeJob& a) { return a.nsym != 0; });
+    }
     for (int f = 0; f < 2; f++) {
         upload_table(S.gather[f], g[f], gp[f]);
         upload_table(S.cg[f], c[f], cp[f]);
@@ -236,7 +246,7 @@ void launch_iteration(amx_server& S)
         const bool f64 = ((kv.first >> 24) & 1) != 0;
         Table& t = kv.second;
         amx::launch_ebe_batched(n, kt, f64, t.dev_jobs.as<amx::EbeJob>(), t.dev_prefix.as<uint32_t>(), t.n, t.total,
-                                s);
+                                s, t.any_nsym);
     }
     if (!S.free_jobs.empty()) {  // element kernels from coordinates (FP64 or FP32) and the job's gather
         const int na = std::min<int>(amx_server::kAux, (int)S.free_jobs.size());

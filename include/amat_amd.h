@@ -207,11 +207,12 @@ typedef struct amx_op_info {
     double penalty;             /* value placed on constrained diagonals */
     uint64_t device_bytes;      /* device memory held by the handle */
     uint64_t k_bytes_fp64, k_bytes_fp32, k_bytes_bf16; /* element-matrix storage */
-    uint64_t alg_bytes_fp64;    /* E(s 8 + 4n) + 16 N, s = values stored per element (n^2, or n(n+1)/2
-                                   packed): one FP64 matvec's algorithmic HBM bytes */
+    uint64_t alg_bytes_fp64;    /* E(b + 4n) + 16 N, b = FP64 bytes stored per element (8 n^2, 8 n(n+1)/2
+                                   packed, the record size of near-symmetric records): one FP64 matvec's
+                                   algorithmic HBM bytes */
     uint64_t flops;             /* 2 sum_e n_e^2 per matvec */
     uint32_t packed_symmetric;  /* element groups stored as packed symmetric tiles */
-    uint32_t reserved;
+    uint32_t near_symmetric;    /* element groups stored as near-symmetric records (FP64) */
 } amx_op_info;
 
 /* aMatFree(Maps&, BC_METH) (include/aMatFree.hpp:405-439).  Copies the maps it
@@ -240,12 +241,17 @@ int amx_op_set_element_function(amx_op* op, int kind, const double* xe, const do
  * streams used by the two-phase (independent/dependent) launch; >= 1 */
 int amx_op_set_num_streams(amx_op* op, uint32_t n_streams);
 
-/* Storage of the element matrices of 8- and 10-DoF groups (no counterpart in
- * the reference, which stores every matrix whole, include/aMatFree.hpp:882-953):
- * mode 1 (default) packs the upper triangles when every matrix of the group is
- * bitwise symmetric -- the kernels then read K[r][c] = K[min(r,c)][max(r,c)]:
- * the same values in the same order, about half the bytes; mode 0 keeps whole
- * matrices.  Takes effect at the next finalize. */
+/* Storage of the element matrices (no counterpart in the reference, which
+ * stores every matrix whole, include/aMatFree.hpp:882-953).  Mode 1 (default):
+ * 8- and 10-DoF groups pack the upper triangles when every matrix of the group
+ * is bitwise symmetric -- the kernels then read K[r][c] = K[min(r,c)][max(r,c)]:
+ * the same values in the same order, about half the bytes; 60-DoF groups keep
+ * their FP64 matrices as near-symmetric records when every matrix is symmetric
+ * up to low-word round-off (upper triangle, one byte per lower entry coding
+ * its low word against the transposed entry, full values for at most a few
+ * hundred positions of the group that the byte cannot code): every entry is
+ * rebuilt bit for bit, about 0.6 of the bytes.  Mode 0 keeps whole matrices.
+ * Takes effect at the next finalize. */
 int amx_op_set_symmetric_storage(amx_op* op, int mode);
 
 /* aMatFree::set_element_matrix -> copy_element_matrix
