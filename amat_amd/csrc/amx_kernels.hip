@@ -448,88 +448,132 @@ template <int B, int E, typename F> __device__ __forceinline__ void static_for(F
     }
 }
 
+// The loads of one record: U and X chunks (all lanes), the lane's delta
+// chunks, its map entry.  Every load is unconditional (indices clamped into the
+// record), so that no wait lands between them.
+template <int N> struct NsymRec {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 ch[nsym_rounds<N>()];
+    u32x4 dq[nsym_nq(N)];
+    uint32_t m;
+
+    __device__ __forceinline__ void issue(const uint8_t* __restrict__ rec, uint32_t nch,
+                                          const uint32_t* __restrict__ map, uint64_t slot)
+    {
+        const uint32_t lane = threadIdx.x & 63u, lc = lane < (uint32_t)N ? lane : (uint32_t)N - 1u;
+        // the map entry first: the u gather waits for it, not for the record
+        m = map[slot * N + lc];
+        __builtin_amdgcn_sched_barrier(0);
+        const u32x4* src = reinterpret_cast<const u32x4*>(rec);
+#pragma unroll
+        for (uint32_t i = 0; i < nsym_rounds<N>(); i++)
+            ch[i] = __builtin_nontemporal_load(src + min(lane + 64u * i, nch - 1u));
+        const u32x4* dsrc = src + nch;
+#pragma unroll
+        for (uint32_t q = 0; q < nsym_nq(N); q++)
+            dq[q] = __builtin_nontemporal_load(dsrc + nsym_dbase(N, q) + (lc >= 16u * q + 1u ? lc - 16u * q - 1u : 0u));
+        // keep these loads ahead of what follows (the u gather, the compute)
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
+    }
+};
+
+template <bool MASKED>
+__device__ __forceinline__ double gather_u(const double* __restrict__ u, uint32_t m)
+{
+    double uv = u[m & kIdxMask];
+    if (MASKED && (m & kFlagBit)) uv = 0.0;
+    return uv;
+}
+
+// copy record R's U and X and the gathered u_e into the wave's LDS region wl
+// (waits for the record); returns the lane's delta chunks, zero for the
+// chunk columns of rows r <= 16q (no lower entry there)
+template <int N>
+__device__ __forceinline__ void nsym_stage(const NsymRec<N>& R, uint32_t nch, uint32_t nxp, double uv, double* wl,
+                                           typename NsymRec<N>::u32x4 (&dq)[nsym_nq(N)])
+{
+    typedef typename NsymRec<N>::u32x4 u32x4;
+    const uint32_t lane = threadIdx.x & 63u, lc = lane < (uint32_t)N ? lane : (uint32_t)N - 1u;
+    u32x4* dst = reinterpret_cast<u32x4*>(wl);
+#pragma unroll
+    for (uint32_t i = 0; i < nsym_rounds<N>(); i++)
+        if (lane + 64u * i < nch) dst[lane + 64u * i] = R.ch[i];
+    wl[nsym_nsp(N) + nxp + lane] = uv;
+#pragma unroll
+    for (uint32_t q = 0; q < nsym_nq(N); q++) dq[q] = lc < 16u * q + 1u ? u32x4{0u, 0u, 0u, 0u} : R.dq[q];
+    wave_lds_sync();
+}
+
+// row r = lc of K_e u_e from the staged record, c ascending.  The LDS word
+// of (r, c) is U[c][r] = off(c) + r - c for c < r and U[r][c] = off(r) + c - r
+// for c >= r (off(a) = a n - a(a-1)/2); the lower form is the smaller of the
+// two while c < r and the larger after (rows shrink by one value each), so
+// the word is min(off(c) - c + r, off(r) - r + c): two adds and a min, with
+// nothing carried from column to column.
+template <int N>
+__device__ __forceinline__ double nsym_row(const typename NsymRec<N>::u32x4 (&dq)[nsym_nq(N)], uint32_t nxp,
+                                           uint32_t xs, uint64_t colany, const double* wl)
+{
+    constexpr uint32_t NSP = nsym_nsp(N);
+    const uint32_t lane = threadIdx.x & 63u, lc = lane < (uint32_t)N ? lane : (uint32_t)N - 1u;
+    const char* kb = reinterpret_cast<const char*>(wl);
+    const double* ush = wl + NSP + nxp;
+    // byte offsets in the region: 8 r and 8 (off(r) - r), the lane's next X value
+    const uint32_t r8 = 8u * lc, b8 = 8u * (nsym_uidx(N, lc, lc) - lc);
+    uint32_t px = 8u * (NSP + xs);
+    double acc = 0.0;
+    static_for<0, N>([&](auto ci) {
+        constexpr int c = decltype(ci)::value;
+        constexpr uint32_t A8 = 8u * (nsym_uidx(N, c, c) - c);
+        const uint32_t wd = dq[c / 16][(c % 16) / 4];
+        const int32_t d = ((int32_t)(wd << (24 - 8 * (c % 4)))) >> 24;
+        // some row has an exception in column c (uniform) and this lane's is one
+        const bool exc = ((colany >> c) & 1u) != 0 && d == -128;
+        const uint32_t a = exc ? px : min(r8 + A8, b8 + 8u * c);
+        px += exc ? 8u : 0u;
+        uint64_t bits = *reinterpret_cast<const uint64_t*>(kb + a);
+        bits = (bits & 0xffffffff00000000ull) | (uint32_t)((uint32_t)bits + (uint32_t)d);
+        acc = madd(acc, __longlong_as_double((long long)bits), ush[c]);
+    });
+    return acc;
+}
+
+// one element: row lc of K_e u_e (m: the lane's map entry)
 template <int N, bool MASKED>
 __device__ __forceinline__ double ebe_nsym_acc(const uint8_t* __restrict__ rec, uint32_t nxp, uint32_t xs,
                                                uint64_t colany, const uint32_t* __restrict__ map,
                                                const double* __restrict__ u, uint64_t slot, double* wl, uint32_t& m)
 {
-    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-    constexpr uint32_t NSP = nsym_nsp(N), NR = nsym_rounds<N>(), NQ = nsym_nq(N);
-    const uint32_t lane = threadIdx.x & 63u;
-    // lanes past N mirror the last row (same addresses, no traffic)
-    const uint32_t lc = lane < (uint32_t)N ? lane : (uint32_t)N - 1u;
-    const uint32_t nch = (NSP + nxp) / 2u;  // 16-byte chunks of U and X
-    // the map entry first (the u gather waits for it, not for the record);
-    // every load unconditional (indices clamped into the record), so that no
-    // wait lands between them
-    m = map[slot * N + lc];
-    __builtin_amdgcn_sched_barrier(0);
-    const u32x4* src = reinterpret_cast<const u32x4*>(rec);
-    u32x4 ch[NR];
-#pragma unroll
-    for (uint32_t i = 0; i < NR; i++) ch[i] = __builtin_nontemporal_load(src + min(lane + 64u * i, nch - 1u));
-    const u32x4* dsrc = src + nch;
-    u32x4 dq[NQ];
-#pragma unroll
-    for (uint32_t q = 0; q < NQ; q++)
-        dq[q] = __builtin_nontemporal_load(dsrc + nsym_dbase(N, q) + (lc >= 16u * q + 1u ? lc - 16u * q - 1u : 0u));
-    // keep the record's loads ahead of the u gather (and unsunk): they are
-    // the long stream, the gather waits only for the map entry
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_sched_barrier(0);
-    double uv = u[m & kIdxMask];
-    if (MASKED && (m & kFlagBit)) uv = 0.0;
-    // rows r <= 16q have no lower entry in chunk column q
-#pragma unroll
-    for (uint32_t q = 0; q < NQ; q++)
-        if (lc < 16u * q + 1u) dq[q] = u32x4{0u, 0u, 0u, 0u};
-    u32x4* dst = reinterpret_cast<u32x4*>(wl);
-#pragma unroll
-    for (uint32_t i = 0; i < NR; i++)
-        if (lane + 64u * i < nch) dst[lane + 64u * i] = ch[i];
-    double* ush = wl + NSP + nxp;
-    ush[lane] = uv;
-    wave_lds_sync();
-    // LDS word indices in the region: the lane's word of column c, its next X value
-    const uint64_t* kw = reinterpret_cast<const uint64_t*>(wl);
-    uint32_t pa = lc, px = NSP + xs;
-    double acc = 0.0;
-    static_for<0, N>([&](auto ci) {
-        constexpr int c = decltype(ci)::value;
-        const uint32_t wd = dq[c / 16][(c % 16) / 4];
-        const int32_t d = ((int32_t)(wd << (24 - 8 * (c % 4)))) >> 24;
-        // some row has an exception in column c (uniform) and this lane's is one
-        const bool exc = ((colany >> c) & 1u) != 0 && d == -128;
-        uint64_t bits = kw[exc ? px : pa];
-        px += exc ? 1u : 0u;
-        bits = (bits & 0xffffffff00000000ull) | (uint32_t)((uint32_t)bits + (uint32_t)d);
-        acc = madd(acc, __longlong_as_double((long long)bits), ush[c]);
-        // lanes r > c (r = lc) step along row c of U, the others along their own row
-        pa += (int)lane > c ? (uint32_t)(N - c - 1) : 1u;
-    });
-    return acc;
+    const uint32_t nch = (nsym_nsp(N) + nxp) / 2u;  // 16-byte chunks of U and X
+    NsymRec<N> R;
+    R.issue(rec, nch, map, slot);
+    m = R.m;
+    typename NsymRec<N>::u32x4 dq[nsym_nq(N)];
+    nsym_stage<N>(R, nch, nxp, gather_u<MASKED>(u, R.m), wl, dq);
+    return nsym_row<N>(dq, nxp, xs, colany, wl);
 }
 
-// one element per wave, blockDim.x / 64 waves per block, dynamic LDS of
-// nsym_region_values(N, nx) doubles per wave
+// one element per wave, one wave per block (more elements per wave with the
+// next record in flight during the compute were slower: the registers they
+// need leave two waves per SIMD instead of the 2-3 the LDS regions allow),
+// dynamic LDS of nsym_region_values(N, nx) doubles
 template <int N, bool MASKED, bool ATOMIC>
-__global__ __launch_bounds__(256) void ebe_nsym_kernel(const uint8_t* __restrict__ K, uint64_t rec_bytes,
-                                                      uint32_t nxp, const uint32_t* __restrict__ xs, uint64_t colany,
-                                                      const uint32_t* __restrict__ map, const double* __restrict__ u,
-                                                      double* __restrict__ out, uint64_t ve_base, uint32_t rs,
-                                                      uint32_t ss, uint32_t tile_begin, uint32_t n_tiles,
-                                                      const uint32_t* __restrict__ skip)
+__global__ __launch_bounds__(64) void ebe_nsym_kernel(const uint8_t* __restrict__ K, uint64_t rec_bytes,
+                                                     uint32_t nxp, const uint32_t* __restrict__ xs, uint64_t colany,
+                                                     const uint32_t* __restrict__ map, const double* __restrict__ u,
+                                                     double* __restrict__ out, uint64_t ve_base, uint32_t rs,
+                                                     uint32_t ss, uint32_t tile_begin, uint32_t n_tiles,
+                                                     const uint32_t* __restrict__ skip)
 {
     extern __shared__ __align__(16) double nsym_lds[];
     if (skip && *skip) return;  // converged solver: nothing to do
-    const uint32_t wpb = blockDim.x >> 6, w = threadIdx.x >> 6;
-    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * wpb + w;
-    if (t >= n_tiles) return;  // uniform per wave
+    const uint32_t t = xcd_block(blockIdx.x, gridDim.x);
+    if (t >= n_tiles) return;
     const uint64_t slot = (uint64_t)tile_begin + t;
-    const uint32_t lane = threadIdx.x & 63u, lc = lane < (uint32_t)N ? lane : (uint32_t)N - 1u;
+    const uint32_t lane = threadIdx.x, lc = lane < (uint32_t)N ? lane : (uint32_t)N - 1u;
     uint32_t m;
-    const double acc = ebe_nsym_acc<N, MASKED>(K + slot * rec_bytes, nxp, xs[lc], colany, map, u, slot,
-                                               nsym_lds + w * (nsym_nsp(N) + nxp + 64u), m);
+    const double acc = ebe_nsym_acc<N, MASKED>(K + slot * rec_bytes, nxp, xs[lc], colany, map, u, slot, nsym_lds, m);
     if (lane < (uint32_t)N) {
         if constexpr (ATOMIC) atomic_add(&out[m & kIdxMask], acc);
         else out[ve_base + (uint64_t)lane * rs + slot * ss] = acc;
@@ -1032,15 +1076,13 @@ void launch_tile(bool masked, bool atomic, bool sym, const EbeArgs& a, hipStream
 #undef AMX_TILE
 }
 
-// waves per block of the near-symmetric kernel
-constexpr uint32_t kNsymWaves = 1;
 
 template <int N>
 void launch_nsym_n(const Layout& L, bool masked, bool atomic, const EbeArgs& a, hipStream_t s)
 {
     const uint32_t nxp = nsym_nxp(L.nx);
-    const dim3 grid(blocks_for(a.n_tiles, kNsymWaves)), block(64 * kNsymWaves);
-    const size_t lds = (size_t)kNsymWaves * nsym_region_values(N, L.nx) * sizeof(double);
+    const dim3 grid(a.n_tiles), block(64);
+    const size_t lds = nsym_region_values(N, L.nx) * sizeof(double);
     const uint8_t* K = static_cast<const uint8_t*>(a.K);
     const uint64_t rb = nsym_record_bytes_of(N, L.nx);
     const double* u = static_cast<const double*>(a.u);
