@@ -240,10 +240,14 @@ void finalize_op(Op& op)
         }
         g.n_slots = (uint32_t)g.slot_elem.size();
         g.n_tiles = g.n_slots / g.G;
-        g.ve_base = ve_rows;
-        ve_rows += (uint64_t)g.n_slots * n;
-        g.ve_rs = g.G > 1 ? g.n_slots : 1u;
-        g.ve_ss = g.G > 1 ? 1u : n;
+        // element rows (ve) of the deterministic scatter; set once the storage
+        // is known (near-symmetric records pad each element's rows to 64)
+        auto assign_ve = [&](uint32_t pitch) {
+            g.ve_base = ve_rows;
+            g.ve_rs = g.G > 1 ? g.n_slots : 1u;
+            g.ve_ss = g.G > 1 ? 1u : pitch;
+            ve_rows += (uint64_t)g.n_slots * (g.G > 1 ? n : pitch);
+        };
 
         // map [slot][r]
         std::vector<uint32_t> hmap((size_t)g.n_slots * n, 0);
@@ -258,6 +262,7 @@ void finalize_op(Op& op)
         }
         upload(g.map, hmap, s, op.device_bytes);
         if (free_m) {
+            assign_ve(n);
             upload_free(op, g, s);
             continue;
         }
@@ -317,6 +322,10 @@ void finalize_op(Op& op)
                 upload(dxrc, xrc, s, tmp);
             }
         }
+        // near-symmetric records: one wave per element stores its n rows; a
+        // 512-byte pitch keeps two elements' rows out of one cache line (whole
+        // lines reach HBM: 214 MB of writes per hex20 64^3 matvec at 480 bytes)
+        assign_ve(g.L[0].nsym ? 64u : n);
         const int bits[3] = {AMX_PREC_FP64, AMX_PREC_FP32, AMX_PREC_BF16};
         for (int kt = 0; kt < 3; kt++) {
             if (kt > 0 && !op.has(bits[kt])) continue;
