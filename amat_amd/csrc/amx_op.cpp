@@ -292,6 +292,10 @@ void finalize_op(Op& op)
         // positions that some matrix cannot code from its transpose; few
         // enough of them -> records, else the whole layout
         DevBuf dxrc;
+        // host sources of the asynchronous uploads below, alive until the
+        // stream synchronization at the end of the group
+        std::vector<uint16_t> xrc, xidx;
+        std::vector<uint32_t> xs;
         if (op.sym_mode == 1 && g.G == 1 && nsym_supported(n) && g.n_slots > 0) {
             DevBuf flags;
             flags.alloc((size_t)n * n * sizeof(uint32_t));
@@ -300,8 +304,8 @@ void finalize_op(Op& op)
             std::vector<uint32_t> hf((size_t)n * n);
             AMX_HIP(hipMemcpyAsync(hf.data(), flags.p, flags.bytes, hipMemcpyDeviceToHost, s));
             AMX_HIP(hipStreamSynchronize(s));
-            std::vector<uint16_t> xrc, xidx((size_t)n * n, 0xffffu);
-            std::vector<uint32_t> xs(n, 0);
+            xidx.assign((size_t)n * n, 0xffffu);
+            xs.assign(n, 0u);
             uint64_t colany = 0;
             for (uint32_t r = 0; r < n; r++) {
                 xs[r] = (uint32_t)xrc.size();
