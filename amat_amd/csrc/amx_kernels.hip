@@ -425,10 +425,9 @@ template <int N, typename KT> constexpr int sym_tiles_per_wave() { return KSize<
 // ascending from U[c][r] plus its delta (c < r: consecutive lanes read
 // consecutive LDS words) or U[r][c] (c >= r): the products and sums of the
 // whole layout in the same order, so the result is bit-identical.  The LDS
-// word of lane r moves by n - c - 1 values from column c to c + 1 while c < r
-// and by one after that, so no address table is needed; columns that hold an
-// exception in some row (colany, the same for every element of the group)
-// also move the lane's cursor over its X values.
+// word of (r, c) is min(off(c) - c + r, off(r) - r + c) (nsym_row); columns
+// that hold an exception in some row (colany, the same for every element of
+// the group) also move the lane's cursor over its X values.
 template <int N> constexpr uint32_t nsym_rounds() { return (nsym_nsp(N) / 2u + 63u) / 64u + 2u; }
 // exception values a record may hold: what the copy rounds hold beyond U
 template <int N> constexpr uint32_t nsym_max_x() { return nsym_rounds<N>() * 128u - nsym_nsp(N); }
@@ -580,6 +579,96 @@ __global__ __launch_bounds__(64) void ebe_nsym_kernel(const uint8_t* __restrict_
     }
 }
 
+// ---------------------------------- packed symmetric records with exceptions
+// FP32 / BF16 storage of the same 60-DoF groups (amx_nsym.hpp): one wave per
+// element, the record (U then X) and u_e staged in the wave's LDS region as
+// for near-symmetric records; lane r reads the value of (r, c) at
+// min(off(c) - c + r, off(r) - r + c) or, where bit c of its row's exception
+// mask is set (a per-group table: mask low, mask high, first X index, per
+// lane), its next X value -- the values and the order of the whole layout
+// (BF16: the zero diagonal slot, then the FP32 diagonal as that layout adds it).
+template <int N, typename KT> constexpr uint32_t psx_rounds()
+{
+    return (nsym_ns(N) * (uint32_t)sizeof(KT) / 16u + 63u) / 64u + 1u;
+}
+template <int N, typename KT> constexpr uint32_t psx_max_x()
+{
+    return psx_rounds<N, KT>() * 64u * 16u / (uint32_t)sizeof(KT) - nsym_ns(N);
+}
+// LDS bytes of one wave's region: the record's values and u_e (64 values of VT)
+template <typename VT>
+__host__ __device__ constexpr uint32_t psx_region_bytes(uint32_t n, uint32_t nx, uint32_t ksize)
+{
+    return psx_record_values(n, nx, ksize) * ksize + 64u * (uint32_t)sizeof(VT);
+}
+
+template <int N, typename KT, typename VT, bool MASKED>
+__device__ __forceinline__ VT psx_acc(const KT* __restrict__ rec, uint32_t nrv, const uint32_t* __restrict__ ptab,
+                                      const uint32_t* __restrict__ map, const VT* __restrict__ u,
+                                      const float* __restrict__ kdiag, uint64_t slot, char* wl, uint32_t& m)
+{
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    constexpr uint32_t NR = psx_rounds<N, KT>(), NS = nsym_ns(N);
+    constexpr bool BF = std::is_same<KT, uint16_t>::value;
+    const uint32_t lane = threadIdx.x & 63u, lc = lane < (uint32_t)N ? lane : (uint32_t)N - 1u;
+    const uint32_t nch = nrv * (uint32_t)sizeof(KT) / 16u;
+    // the map entry first (the u gather waits for it), then the record
+    m = map[slot * N + lc];
+    __builtin_amdgcn_sched_barrier(0);
+    const u32x4* src = reinterpret_cast<const u32x4*>(rec);
+    u32x4 ch[NR];
+#pragma unroll
+    for (uint32_t i = 0; i < NR; i++) ch[i] = __builtin_nontemporal_load(src + min(lane + 64u * i, nch - 1u));
+    const uint32_t mlo = ptab[3u * lane], mhi = ptab[3u * lane + 1u], xs = ptab[3u * lane + 2u];
+    float kd = 0.0f;
+    if constexpr (BF) kd = kdiag[slot * N + lc];
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    VT uv = u[m & kIdxMask];
+    if (MASKED && (m & kFlagBit)) uv = VT(0);
+    u32x4* dst = reinterpret_cast<u32x4*>(wl);
+#pragma unroll
+    for (uint32_t i = 0; i < NR; i++)
+        if (lane + 64u * i < nch) dst[lane + 64u * i] = ch[i];
+    VT* ush = reinterpret_cast<VT*>(wl + 16u * nch);
+    ush[lane] = uv;
+    wave_lds_sync();
+    const KT* kv = reinterpret_cast<const KT*>(wl);
+    const uint32_t b = nsym_uidx(N, lc, lc) - lc;  // off(r) - r
+    uint32_t px = NS + xs;
+    VT acc = VT(0);
+    static_for<0, N>([&](auto ci) {
+        constexpr int c = decltype(ci)::value;
+        const bool exc = (((c < 32 ? mlo >> (c & 31) : mhi >> (c & 31))) & 1u) != 0;
+        const uint32_t a = exc ? px : min(lc + (nsym_uidx(N, c, c) - c), b + c);
+        px += exc ? 1u : 0u;
+        acc = madd(acc, (VT)kval(kv[a]), ush[c]);
+    });
+    if constexpr (BF) acc = madd(acc, (VT)kd, uv);
+    return acc;
+}
+
+template <int N, typename KT, typename VT, bool MASKED, bool ATOMIC>
+__global__ __launch_bounds__(64) void psx_kernel(const KT* __restrict__ K, uint32_t nrv,
+                                                const uint32_t* __restrict__ ptab, const uint32_t* __restrict__ map,
+                                                const VT* __restrict__ u, VT* __restrict__ out, uint64_t ve_base,
+                                                uint32_t rs, uint32_t ss, uint32_t tile_begin, uint32_t n_tiles,
+                                                const uint32_t* __restrict__ skip, const float* __restrict__ kdiag)
+{
+    extern __shared__ __align__(16) char psx_lds[];
+    if (skip && *skip) return;  // converged solver: nothing to do
+    const uint32_t t = xcd_block(blockIdx.x, gridDim.x);
+    if (t >= n_tiles) return;
+    const uint64_t slot = (uint64_t)tile_begin + t;
+    const uint32_t lane = threadIdx.x;
+    uint32_t m;
+    const VT acc = psx_acc<N, KT, VT, MASKED>(K + slot * nrv, nrv, ptab, map, u, kdiag, slot, psx_lds, m);
+    if (lane < (uint32_t)N) {
+        if constexpr (ATOMIC) atomic_add(&out[m & kIdxMask], acc);
+        else out[ve_base + (uint64_t)lane * rs + slot * ss] = acc;
+    }
+}
+
 // index of the entry that owns work item x: the largest j < n with prefix[j] <= x
 // (x wave-uniform, so the search runs on the scalar unit)
 __device__ __forceinline__ uint32_t owner_of(const uint32_t* __restrict__ prefix, uint32_t n, uint32_t x)
@@ -607,6 +696,21 @@ __global__ __launch_bounds__(256) void ebe_tile_batched_kernel(const EbeJob* __r
     const EbeJob& jb = jobs[j];
     if (jb.skip && *jb.skip) return;
     const uint32_t t = jb.tile_begin + (gt - prefix[j]);
+    if constexpr (N == 60 && !std::is_same<KT, double>::value) {
+        if (jb.nsym) {  // packed symmetric records with exceptions: one region per wave
+            extern __shared__ __align__(16) char psx_lds[];
+            const uint32_t nrv = psx_record_values(60, jb.nsym_nx, (uint32_t)sizeof(KT)), lane = threadIdx.x & 63u;
+            char* wl = psx_lds + (threadIdx.x >> 6) * psx_region_bytes<VT>(60, psx_max_x<60, KT>(), sizeof(KT));
+            uint32_t m;
+            const KT* rec = static_cast<const KT*>(jb.K) + (uint64_t)t * nrv;
+            const VT acc = jb.masked ? psx_acc<60, KT, VT, true>(rec, nrv, jb.nsym_xs, jb.map,
+                                                                 static_cast<const VT*>(jb.u), jb.kdiag, t, wl, m)
+                                     : psx_acc<60, KT, VT, false>(rec, nrv, jb.nsym_xs, jb.map,
+                                                                  static_cast<const VT*>(jb.u), jb.kdiag, t, wl, m);
+            if (lane < 60u) static_cast<VT*>(jb.out)[jb.ve_base + (uint64_t)lane * jb.ve_rs + (uint64_t)t * jb.ve_ss] = acc;
+            return;
+        }
+    }
     if constexpr (N == 60 && std::is_same<KT, double>::value && std::is_same<VT, double>::value) {
         if (jb.nsym) {  // near-symmetric records: one region per wave in the launch's dynamic LDS
             extern __shared__ __align__(16) double nsym_lds[];
@@ -866,10 +970,12 @@ __device__ __forceinline__ bool nsym_codable(double lower, double upper, int32_t
 }
 
 // flags[r*N + c] = 1 for the lower entries that some matrix of the group
-// cannot code (one thread per slot and lower entry; every writer stores 1)
+// cannot code (one thread per slot and lower entry; every writer stores 1):
+// kt F64 -- the byte code of near-symmetric records; F32 / BF16 -- the
+// rounded entry differs from the rounded transposed entry
 __global__ void nsym_scan_kernel(const double* __restrict__ src, const double* __restrict__ kref,
                                  const double* __restrict__ scale, const uint32_t* __restrict__ slot_elem,
-                                 uint64_t total, uint32_t N, uint32_t* __restrict__ flags)
+                                 uint64_t total, uint32_t N, int kt, uint32_t* __restrict__ flags)
 {
     const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= total) return;
@@ -879,9 +985,48 @@ __global__ void nsym_scan_kernel(const double* __restrict__ src, const double* _
     if (e == kPadSlot) return;
     uint32_t r, c;
     lower_rc((uint32_t)(idx - slot * NL), r, c);
-    int32_t d;
-    if (!nsym_codable(src_value(src, kref, scale, e, N, r, c), src_value(src, kref, scale, e, N, c, r), d))
-        flags[r * N + c] = 1u;
+    const double lo = src_value(src, kref, scale, e, N, r, c), up = src_value(src, kref, scale, e, N, c, r);
+    bool ok;
+    if (kt == (int)KType::F32) {
+        ok = __float_as_uint(from_f64<float>(lo)) == __float_as_uint(from_f64<float>(up));
+    } else if (kt == (int)KType::BF16) {
+        ok = from_f64<uint16_t>(lo) == from_f64<uint16_t>(up);
+    } else {
+        int32_t d;
+        ok = nsym_codable(lo, up, d);
+    }
+    if (!ok) flags[r * N + c] = 1u;
+}
+
+// packed symmetric records with exceptions, one thread per stored value;
+// xrc[x] = r*N + c of exception x
+template <typename KT>
+__global__ void relayout_psx_kernel(const double* __restrict__ src, const double* __restrict__ kref,
+                                    const double* __restrict__ scale, const uint32_t* __restrict__ slot_elem,
+                                    uint64_t total, uint32_t N, uint32_t nx, uint32_t nrv,
+                                    const uint16_t* __restrict__ xrc, KT* __restrict__ dst)
+{
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const uint64_t slot = idx / nrv;
+    const uint32_t v = (uint32_t)(idx - slot * nrv);
+    const uint32_t e = slot_elem[slot];
+    double val = 0.0;
+    if (e != kPadSlot) {
+        if (v < nsym_ns(N)) {
+            uint32_t a = 0, k = v;
+            while (k >= N - a) {  // row a of the upper triangle holds N - a values
+                k -= N - a;
+                a++;
+            }
+            // BF16: the diagonal is kept in FP32 (bf16_diag_kernel), its slot is 0
+            if (!(std::is_same<KT, uint16_t>::value && k == 0)) val = src_value(src, kref, scale, e, N, a, a + k);
+        } else if (v < nsym_ns(N) + nx) {
+            const uint32_t rc = xrc[v - nsym_ns(N)];
+            val = src_value(src, kref, scale, e, N, rc / N, rc % N);
+        }
+    }
+    dst[idx] = from_f64<KT>(val);
 }
 
 // records, one thread per 32-bit word.  xrc[x]: r*N + c of exception x;
@@ -1100,6 +1245,28 @@ void launch_nsym_n(const Layout& L, bool masked, bool atomic, const EbeArgs& a, 
 #undef AMX_NSYM
 }
 
+template <int N, typename KT, typename VT>
+void launch_psx(const Layout& L, bool masked, bool atomic, const EbeArgs& a, hipStream_t s)
+{
+    const uint32_t nrv = psx_record_values(N, L.nx, sizeof(KT));
+    const dim3 grid(a.n_tiles), block(64);
+    const size_t lds = psx_region_bytes<VT>(N, L.nx, sizeof(KT));
+    const KT* K = static_cast<const KT*>(a.K);
+    const VT* u = static_cast<const VT*>(a.u);
+    VT* out = static_cast<VT*>(a.out);
+#define AMX_PSX(M, A)                                                                                              \
+    psx_kernel<N, KT, VT, M, A><<<grid, block, lds, s>>>(K, nrv, a.nsym_xs, a.map, u, out, a.ve_base, a.ve_rs,      \
+                                                         a.ve_ss, a.tile_begin, a.n_tiles, a.skip, a.kdiag)
+    if (masked) {
+        if (atomic) AMX_PSX(true, true);
+        else AMX_PSX(true, false);
+    } else {
+        if (atomic) AMX_PSX(false, true);
+        else AMX_PSX(false, false);
+    }
+#undef AMX_PSX
+}
+
 template <typename KT, typename VT>
 void launch_generic(uint32_t n, bool masked, bool atomic, const EbeArgs& a, hipStream_t s)
 {
@@ -1150,17 +1317,18 @@ bool sym_supported(uint32_t n) { return n == 8 || n == 10; }
 
 bool nsym_supported(uint32_t n) { return n == 60; }
 
-uint32_t nsym_max_exceptions(uint32_t n)
+uint32_t nsym_max_exceptions(uint32_t n, KType kt)
 {
     AMX_REQUIRE(nsym_supported(n), AMX_INVALID_ARGUMENT, "no near-symmetric kernel for n = " + std::to_string(n));
-    return nsym_max_x<60>();
+    return kt == KType::F64 ? nsym_max_x<60>() : kt == KType::F32 ? psx_max_x<60, float>() : psx_max_x<60, uint16_t>();
 }
 
 uint64_t nsym_record_bytes(uint32_t n, uint32_t nx) { return nsym_record_bytes_of(n, nx); }
 
 uint64_t tile_values(const Layout& L, KType kt)
 {
-    if (L.nsym) return nsym_record_bytes_of(L.n, L.nx) / ktype_size(kt);
+    if (L.nsym && kt == KType::F64) return nsym_record_bytes_of(L.n, L.nx) / 8u;
+    if (L.nsym) return psx_record_values(L.n, L.nx, (uint32_t)ktype_size(kt));
     if (L.sym) return sym_tile_values(L.n, (uint32_t)ktype_size(kt));
     return (uint64_t)L.G * L.n * L.n;
 }
@@ -1240,11 +1408,27 @@ void launch_bf16_diag(const double* src_rowmajor, const double* k_ref, const dou
 }
 
 void launch_nsym_scan(const double* src, const double* k_ref, const double* scale, const uint32_t* slot_elem,
-                      uint32_t n_slots, uint32_t n, uint32_t* flags, hipStream_t s)
+                      uint32_t n_slots, uint32_t n, KType kt, uint32_t* flags, hipStream_t s)
 {
     const uint64_t total = (uint64_t)n_slots * (n * (n - 1u) / 2u);
     if (total == 0) return;
-    nsym_scan_kernel<<<blocks_for(total, 256), 256, 0, s>>>(src, k_ref, scale, slot_elem, total, n, flags);
+    nsym_scan_kernel<<<blocks_for(total, 256), 256, 0, s>>>(src, k_ref, scale, slot_elem, total, n, (int)kt, flags);
+    AMX_HIP(hipGetLastError());
+}
+
+void launch_relayout_psx(const double* src, const double* k_ref, const double* scale, const uint32_t* slot_elem,
+                         uint32_t n_slots, const Layout& L, KType kt, const uint16_t* xrc, void* dst, hipStream_t s)
+{
+    const uint32_t nrv = psx_record_values(L.n, L.nx, (uint32_t)ktype_size(kt));
+    const uint64_t total = (uint64_t)n_slots * nrv;
+    if (total == 0) return;
+    const uint32_t nb = blocks_for(total, 256);
+    if (kt == KType::F32)
+        relayout_psx_kernel<float><<<nb, 256, 0, s>>>(src, k_ref, scale, slot_elem, total, L.n, L.nx, nrv, xrc,
+                                                      static_cast<float*>(dst));
+    else
+        relayout_psx_kernel<uint16_t><<<nb, 256, 0, s>>>(src, k_ref, scale, slot_elem, total, L.n, L.nx, nrv, xrc,
+                                                         static_cast<uint16_t*>(dst));
     AMX_HIP(hipGetLastError());
 }
 
@@ -1279,9 +1463,13 @@ void launch_ebe(const Layout& L, KType kt, bool vt_double, bool masked, bool ato
 {
     if (a.n_tiles == 0) return;
     if (L.nsym) {
-        AMX_REQUIRE(kt == KType::F64 && vt_double && L.n == 60, AMX_INVALID_STATE,
-                    "near-symmetric records hold FP64 matrices of 60 DoFs");
-        launch_nsym_n<60>(L, masked, atomic, a, s);
+        AMX_REQUIRE(L.n == 60 && (kt != KType::F64 || vt_double), AMX_INVALID_STATE,
+                    "near-symmetric records hold matrices of 60 DoFs");
+        if (kt == KType::F64) launch_nsym_n<60>(L, masked, atomic, a, s);
+        else if (kt == KType::F32 && vt_double) launch_psx<60, float, double>(L, masked, atomic, a, s);
+        else if (kt == KType::F32) launch_psx<60, float, float>(L, masked, atomic, a, s);
+        else if (vt_double) launch_psx<60, uint16_t, double>(L, masked, atomic, a, s);
+        else launch_psx<60, uint16_t, float>(L, masked, atomic, a, s);
         AMX_HIP(hipGetLastError());
         return;
     }
@@ -1328,11 +1516,24 @@ void launch_ebe_batched(uint32_t n, KType kt, bool vt_double, const EbeJob* jobs
     // near-symmetric records: four regions of the largest record a job may have
     size_t lds = 0;
     if (any_nsym) {
-        AMX_REQUIRE(n == 60 && kt == KType::F64 && vt_double, AMX_INVALID_STATE,
-                    "near-symmetric records hold FP64 matrices of 60 DoFs");
-        lds = 4u * nsym_region_values(60, nsym_max_x<60>()) * sizeof(double);
-        AMX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&ebe_tile_batched_kernel<60, double, double>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        AMX_REQUIRE(n == 60 && (kt != KType::F64 || vt_double), AMX_INVALID_STATE,
+                    "near-symmetric records hold matrices of 60 DoFs");
+        const void* fn = nullptr;
+        if (kt == KType::F64) {
+            lds = 4u * nsym_region_values(60, nsym_max_x<60>()) * sizeof(double);
+            fn = reinterpret_cast<const void*>(&ebe_tile_batched_kernel<60, double, double>);
+        } else if (kt == KType::F32) {
+            lds = 4u * (vt_double ? psx_region_bytes<double>(60, psx_max_x<60, float>(), 4)
+                                  : psx_region_bytes<float>(60, psx_max_x<60, float>(), 4));
+            fn = vt_double ? reinterpret_cast<const void*>(&ebe_tile_batched_kernel<60, float, double>)
+                           : reinterpret_cast<const void*>(&ebe_tile_batched_kernel<60, float, float>);
+        } else {
+            lds = 4u * (vt_double ? psx_region_bytes<double>(60, psx_max_x<60, uint16_t>(), 2)
+                                  : psx_region_bytes<float>(60, psx_max_x<60, uint16_t>(), 2));
+            fn = vt_double ? reinterpret_cast<const void*>(&ebe_tile_batched_kernel<60, uint16_t, double>)
+                           : reinterpret_cast<const void*>(&ebe_tile_batched_kernel<60, uint16_t, float>);
+        }
+        AMX_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
 #define AMX_EBE_B(NN, KTT, VTT) ebe_tile_batched_kernel<NN, KTT, VTT><<<nb, 256, lds, s>>>(jobs, prefix, n_jobs)
 #define AMX_EBE_N(KTT, VTT)                                                                                        \

@@ -24,6 +24,10 @@ enum class KType : int { F64 = 0, F32 = 1, BF16 = 2 };
 // to 16 bytes.  The kernels read K[r][c] = K[min][max]: the same values in the
 // same order as the full layout, about half the bytes.
 //
+// FP32 / BF16 storage of such a group (nsym in L[1] / L[2]) holds packed
+// symmetric records with exceptions: the rounded matrices are bitwise
+// symmetric except at the group's exception positions (amx_nsym.hpp).
+//
 // Near-symmetric records (nsym, one-element tiles of FP64 storage, n = 60):
 // when every matrix of the group is symmetric up to small differences in the
 // low word of the transposed entries (quadrature round-off) and at most
@@ -45,7 +49,7 @@ size_t ktype_size(KType kt);
 bool sym_supported(uint32_t n);
 // near-symmetric records exist for these DoF counts (one-element FP64 tiles)
 bool nsym_supported(uint32_t n);
-uint32_t nsym_max_exceptions(uint32_t n);
+uint32_t nsym_max_exceptions(uint32_t n, KType kt);
 uint64_t nsym_record_bytes(uint32_t n, uint32_t nx);
 // values per stored tile of layout L in storage kt (padding included)
 uint64_t tile_values(const Layout& L, KType kt);
@@ -66,7 +70,10 @@ void launch_bf16_diag(const double* src_rowmajor, const double* k_ref, const dou
 // the group cannot code K[r][c] from K[c][r] (flags zeroed by the caller);
 // source either row-major matrices (src) or k_ref + scale
 void launch_nsym_scan(const double* src, const double* k_ref, const double* scale, const uint32_t* slot_elem,
-                      uint32_t n_slots, uint32_t n, uint32_t* flags, hipStream_t s);
+                      uint32_t n_slots, uint32_t n, KType kt, uint32_t* flags, hipStream_t s);
+// packed symmetric records with exceptions of FP32 / BF16 storage (L.nsym for kt F32 / BF16)
+void launch_relayout_psx(const double* src, const double* k_ref, const double* scale, const uint32_t* slot_elem,
+                         uint32_t n_slots, const Layout& L, KType kt, const uint16_t* xrc, void* dst, hipStream_t s);
 // the records of layout L (L.nsym, L.nx): xrc[x] = r*n + c of exception x in
 // row-major order, xidx[r*n + c] = x (0xffff elsewhere)
 void launch_relayout_nsym(const double* src, const double* k_ref, const double* scale, const uint32_t* slot_elem,
@@ -88,8 +95,9 @@ struct EbeArgs {
     uint32_t tile_begin, n_tiles;
     const uint32_t* skip = nullptr;  // device flag: nonzero -> kernel exits (converged solver)
     const float* kdiag = nullptr;    // BF16 storage: FP32 diagonal [slot][r]
-    const uint32_t* nsym_xs = nullptr;  // near-symmetric records: first X index of each row
-    uint64_t nsym_colany = 0;           // bit c: some row has an exception in column c
+    const uint32_t* nsym_xs = nullptr;  // near-symmetric records: first X index of each row (FP64); FP32 /
+                                        // BF16: per lane exception mask low, high, first X index
+    uint64_t nsym_colany = 0;           // FP64: bit c: some row has an exception in column c
 };
 // vt_double: vectors/accumulation in FP64 (only with KType::F64), else FP32.
 void launch_ebe(const Layout& L, KType kt, bool vt_double, bool masked, bool atomic, const EbeArgs& a,

@@ -51,6 +51,17 @@ __host__ __device__ constexpr uint32_t nsym_uidx(uint32_t n, uint32_t a, uint32_
     return a * n - (a * (a - 1u)) / 2u + (b - a);
 }
 
+// Packed symmetric records with exceptions (FP32 / BF16 storage of the same
+// groups): rounded to FP32 or BF16, such matrices are bitwise symmetric except
+// at their exception positions, so a record holds U (the BF16 diagonal slot
+// is 0: that storage keeps its diagonal in FP32 beside the matrices) and then
+// the group's nx exception values of the storage type, in row-major order of
+// their (r, c), padded to 16 bytes.
+__host__ __device__ constexpr uint32_t psx_record_values(uint32_t n, uint32_t nx, uint32_t ksize)
+{
+    return ((nsym_ns(n) + nx) * ksize + 15u) / 16u * 16u / ksize;
+}
+
 // one value K[r][c] of a record (set-up kernels; the element kernel reads the
 // record through LDS).  xidx[r*n + c]: index into X of an exception, else any.
 __device__ __forceinline__ double nsym_value(const double* __restrict__ rec, uint32_t n, uint32_t nx,

@@ -179,6 +179,7 @@ void finalize_op(Op& op)
         g.nsym_xs.release();
         g.nsym_xidx.release();
         g.nsym_colany = 0;
+        for (auto& t : g.psx_tab) t.release();
     }
     const bool free_m = op.is_free();
     if (free_m) {
@@ -291,46 +292,66 @@ void finalize_op(Op& op)
         // near-symmetric records for one-element FP64 tiles: find the lower
         // positions that some matrix cannot code from its transpose; few
         // enough of them -> records, else the whole layout
-        DevBuf dxrc;
+        // near-symmetric records (FP64) and packed symmetric records with
+        // exceptions (FP32 / BF16) for one-element tiles: find the lower
+        // positions that some matrix cannot take from its transposed entry;
+        // few enough of them -> records, else the whole layout
+        const int bits[3] = {AMX_PREC_FP64, AMX_PREC_FP32, AMX_PREC_BF16};
+        DevBuf dxrc[3];
         // host sources of the asynchronous uploads below, alive until the
         // stream synchronization at the end of the group
-        std::vector<uint16_t> xrc, xidx;
-        std::vector<uint32_t> xs;
-        if (op.sym_mode == 1 && g.G == 1 && nsym_supported(n) && g.n_slots > 0) {
+        std::vector<uint16_t> xrc[3], xidx;
+        std::vector<uint32_t> xs, ptab[3];
+        for (int kt = 0; kt < 3; kt++) {
+            if (op.sym_mode != 1 || g.G != 1 || !nsym_supported(n) || g.n_slots == 0) break;
+            if (kt > 0 && !op.has(bits[kt])) continue;
             DevBuf flags;
             flags.alloc((size_t)n * n * sizeof(uint32_t));
             AMX_HIP(hipMemsetAsync(flags.p, 0, flags.bytes, s));
-            launch_nsym_scan(src_rm, src_kref, src_sc, dslot.as<uint32_t>(), g.n_slots, n, flags.as<uint32_t>(), s);
+            launch_nsym_scan(src_rm, src_kref, src_sc, dslot.as<uint32_t>(), g.n_slots, n, (KType)kt,
+                             flags.as<uint32_t>(), s);
             std::vector<uint32_t> hf((size_t)n * n);
             AMX_HIP(hipMemcpyAsync(hf.data(), flags.p, flags.bytes, hipMemcpyDeviceToHost, s));
             AMX_HIP(hipStreamSynchronize(s));
-            xidx.assign((size_t)n * n, 0xffffu);
-            xs.assign(n, 0u);
+            std::vector<uint32_t> first(n, 0u);
             uint64_t colany = 0;
+            std::vector<uint64_t> rowmask(n, 0ull);
+            if (kt == 0) xidx.assign((size_t)n * n, 0xffffu);
             for (uint32_t r = 0; r < n; r++) {
-                xs[r] = (uint32_t)xrc.size();
+                first[r] = (uint32_t)xrc[kt].size();
                 for (uint32_t c = 0; c < r; c++) {
                     if (!hf[(size_t)r * n + c]) continue;
-                    xidx[(size_t)r * n + c] = (uint16_t)xrc.size();
-                    xrc.push_back((uint16_t)(r * n + c));
+                    if (kt == 0) xidx[(size_t)r * n + c] = (uint16_t)xrc[kt].size();
+                    xrc[kt].push_back((uint16_t)(r * n + c));
                     colany |= 1ull << c;
+                    rowmask[r] |= 1ull << c;
                 }
             }
-            if (xrc.size() <= nsym_max_exceptions(n)) {
-                g.L[0].nsym = true;
-                g.L[0].nx = (uint32_t)xrc.size();
+            if (xrc[kt].size() > nsym_max_exceptions(n, (KType)kt)) continue;
+            g.L[kt].nsym = true;
+            g.L[kt].nx = (uint32_t)xrc[kt].size();
+            uint64_t tmp = 0;
+            upload(dxrc[kt], xrc[kt], s, tmp);
+            if (kt == 0) {
                 g.nsym_colany = colany;
+                xs = first;
                 upload(g.nsym_xs, xs, s, op.device_bytes);
                 upload(g.nsym_xidx, xidx, s, op.device_bytes);
-                uint64_t tmp = 0;
-                upload(dxrc, xrc, s, tmp);
+            } else {  // per lane (row min(l, n-1)): exception mask low, high, first X index
+                ptab[kt].assign(3 * 64, 0u);
+                for (uint32_t l = 0; l < 64; l++) {
+                    const uint32_t r = std::min(l, n - 1u);
+                    ptab[kt][3 * l] = (uint32_t)rowmask[r];
+                    ptab[kt][3 * l + 1] = (uint32_t)(rowmask[r] >> 32);
+                    ptab[kt][3 * l + 2] = first[r];
+                }
+                upload(g.psx_tab[kt], ptab[kt], s, op.device_bytes);
             }
         }
         // near-symmetric records: one wave per element stores its n rows; a
         // 512-byte pitch keeps two elements' rows out of one cache line (whole
         // lines reach HBM: 214 MB of writes per hex20 64^3 matvec at 480 bytes)
         assign_ve(g.L[0].nsym ? 64u : n);
-        const int bits[3] = {AMX_PREC_FP64, AMX_PREC_FP32, AMX_PREC_BF16};
         for (int kt = 0; kt < 3; kt++) {
             if (kt > 0 && !op.has(bits[kt])) continue;
             const uint64_t bytes = (uint64_t)g.n_tiles * tile_values(g.L[kt], (KType)kt) * ktype_size((KType)kt);
@@ -338,7 +359,10 @@ void finalize_op(Op& op)
             op.device_bytes += g.K[kt].bytes;
             if (kt == 0 && g.L[0].nsym)
                 launch_relayout_nsym(src_rm, src_kref, src_sc, dslot.as<uint32_t>(), g.n_slots, g.L[0],
-                                     dxrc.as<uint16_t>(), g.nsym_xidx.as<uint16_t>(), g.K[0].p, s);
+                                     dxrc[0].as<uint16_t>(), g.nsym_xidx.as<uint16_t>(), g.K[0].p, s);
+            else if (g.L[kt].nsym)
+                launch_relayout_psx(src_rm, src_kref, src_sc, dslot.as<uint32_t>(), g.n_slots, g.L[kt], (KType)kt,
+                                    dxrc[kt].as<uint16_t>(), g.K[kt].p, s);
             else if (op.scaled)
                 launch_relayout_scaled(src.as<double>(), src_scale.as<double>(), dslot.as<uint32_t>(), g.n_slots,
                                        g.L[kt], (KType)kt, g.K[kt].p, s);
@@ -578,7 +602,7 @@ void ebe_launch(Op& op, KType kt, bool vt_double, bool masked, int phase, const 
         a.ve_ss = g.ve_ss;
         a.skip = skip;
         a.kdiag = kt == KType::BF16 ? g.kdiag16.as<float>() : nullptr;
-        a.nsym_xs = g.nsym_xs.as<uint32_t>();
+        a.nsym_xs = k == 0 ? g.nsym_xs.as<uint32_t>() : g.psx_tab[k].as<uint32_t>();
         a.nsym_colany = g.nsym_colany;
         if (phase == 0) { a.tile_begin = 0; a.n_tiles = g.tiles_phase0; }
         else if (phase == 1) { a.tile_begin = g.tiles_phase0; a.n_tiles = g.n_tiles - g.tiles_phase0; }
@@ -904,11 +928,16 @@ int amx_op_get_info(const amx_op* h, amx_op_info* info)
             // tiles; near-symmetric records (FP64 only) by their record size
             const uint64_t ns = g.L[0].sym ? (uint64_t)g.n * (g.n + 1) / 2 : nn;
             const uint64_t b64 = g.L[0].nsym ? amx::nsym_record_bytes(g.n, g.L[0].nx) : ns * 8;
+            // FP32 / BF16: the records as stored when built, else what they would hold
+            auto bytes_of = [&](int kt) {
+                return g.L[kt].nsym ? amx::tile_values(g.L[kt], (KType)kt) * amx::ktype_size((KType)kt)
+                                    : ns * amx::ktype_size((KType)kt);
+            };
             if (g.L[0].sym) info->packed_symmetric++;
             if (g.L[0].nsym) info->near_symmetric++;
             info->k_bytes_fp64 += ne * b64;
-            info->k_bytes_fp32 += ne * ns * 4;
-            info->k_bytes_bf16 += ne * ns * 2;
+            info->k_bytes_fp32 += ne * bytes_of(1);
+            info->k_bytes_bf16 += ne * bytes_of(2);
             info->alg_bytes_fp64 += ne * (b64 + 4ull * g.n);
             info->flops += 2 * ne * nn;
         }

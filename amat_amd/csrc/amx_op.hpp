@@ -37,6 +37,9 @@ struct Group {
     // exception index of each (r, c) (0xffff: none), columns holding one
     DevBuf nsym_xs, nsym_xidx;
     uint64_t nsym_colany = 0;
+    // FP32 / BF16 records (L[1] / L[2].nsym): per lane exception mask low,
+    // high and first X index ([kt], kt = 1, 2)
+    DevBuf psx_tab[3];
 };
 
 struct Op {

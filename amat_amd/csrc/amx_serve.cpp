@@ -187,7 +187,7 @@ void rebuild_tables(amx_server& S)
             a.sym = grp.L[(int)j.kt].sym ? 1u : 0u;
             a.nsym = grp.L[(int)j.kt].nsym ? 1u : 0u;
             a.nsym_nx = grp.L[(int)j.kt].nx;
-            a.nsym_xs = grp.nsym_xs.as<uint32_t>();
+            a.nsym_xs = j.kt == KType::F64 ? grp.nsym_xs.as<uint32_t>() : grp.psx_tab[(int)j.kt].as<uint32_t>();
             a.nsym_colany = grp.nsym_colany;
             e.first.push_back(a);
             e.second.push_back(e.second.back() + grp.n_tiles);

@@ -169,3 +169,55 @@ def test_multi_rhs_layout_from_records(torch_gpu):
         mr.apply_dev(nrhs, [dU], [dV], torch_gpu.cuda.current_stream())
         outs.append(dV.cpu().numpy())
     np.testing.assert_array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("prec", [PREC.FP32, PREC.BF16])
+@pytest.mark.parametrize("case", ["elasticity", "synthetic", "exact", "too_many"])
+def test_low_precision_records(torch_gpu, prec, case):
+    """FP32 / BF16 storage of 60-DoF groups: rounded, the matrices are bitwise
+    symmetric except at the group's exception positions, so the records hold
+    the upper triangle and those values; products bit-identical to the whole
+    layout with FP32 vectors, FP64 vectors (the inner solves of
+    AMX_IR_F64_VECTORS, through the solver's result) and the atomic scatter
+    within tolerance."""
+    p = problems.elasticity_hex20(4, 3, 2)
+    if case == "synthetic":
+        p = with_matrices(p, near_symmetric(21, p.n_elem, 57))
+    elif case == "exact":      # exactly symmetric: no exception at all
+        K = p.element_matrices()
+        p = with_matrices(p, K + K.transpose(0, 2, 1))
+    elif case == "too_many":   # more exceptions than a record holds: whole matrices
+        rng = np.random.default_rng(5)
+        p = with_matrices(p, rng.normal(size=(p.n_elem, N, N)))
+    kw = dict(precision=PREC.FP64 | PREC.FP32 | PREC.BF16)
+    rec, whole = ops_pair(p, **kw)
+    ir, iw = rec.info(), whole.info()
+    kb = "k_bytes_fp32" if prec == PREC.FP32 else "k_bytes_bf16"
+    if case == "too_many":
+        assert getattr(ir, kb) == getattr(iw, kb)
+    else:
+        assert getattr(ir, kb) < 0.56 * getattr(iw, kb)
+    u = p.random_u()
+    for fn in ("matmult_f32_dev",):
+        a = run(torch_gpu, lambda v, uu, s: rec.matmult_f32_dev(prec, v, uu, s), u, np.float32)
+        b = run(torch_gpu, lambda v, uu, s: whole.matmult_f32_dev(prec, v, uu, s), u, np.float32)
+        np.testing.assert_array_equal(a, b)
+    ato = p.build_operator(aMatFree, Maps, sym_storage=1, scatter=SCATTER.ATOMIC, **kw)
+    a = run(torch_gpu, lambda v, uu, s: ato.matmult_f32_dev(prec, v, uu, s), u, np.float32)
+    b = run(torch_gpu, lambda v, uu, s: whole.matmult_f32_dev(prec, v, uu, s), u, np.float32)
+    assert rel_l2(a, b) <= 1e-6
+    # FP64 inner vectors with FP32 / BF16 storage (SPD cases): the solves take the same path bit for bit
+    from amat_amd.amat import IR_F64_VECTORS
+    if case in ("synthetic", "too_many"):
+        return
+    b64 = p.random_u(seed=9)
+    b64[p.constrained.astype(np.int64)] = 0.0
+    xs = []
+    for op in (rec, whole):
+        db = torch_gpu.from_numpy(b64).cuda()
+        dx = torch_gpu.zeros_like(db)
+        st = op.ir_cg(int(prec) | IR_F64_VECTORS, db, dx, rtol=1e-8, inner_rtol=1e-2, max_outer=30, max_inner=3000,
+                      stream=torch_gpu.cuda.current_stream())
+        xs.append((dx.cpu().numpy(), st.iterations))
+    np.testing.assert_array_equal(xs[0][0], xs[1][0])
+    assert xs[0][1] == xs[1][1]
