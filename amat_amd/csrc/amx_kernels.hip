@@ -790,34 +790,54 @@ __global__ __launch_bounds__(256) void ebe_generic_kernel(const KT* __restrict__
 }
 
 // ------------------------------------------------------------------ gather
+// A block owns 256 consecutive DoFs d0 .. d0+255, whose lists are the
+// contiguous entry range [off[d0], off[d0+256]).  The block reads that range
+// in windows of kGatherWindow entries: every thread loads entries tid,
+// tid + 256, ... of the window (list index, then element row) with all loads
+// in flight together, and the rows land in LDS; then each thread adds the
+// part of its own DoF's list inside the window, in list order, to its running
+// sum.  The loads no longer follow one list per lane (no dependent chains, no
+// lanes idle while a neighbour walks a long list), and every sum keeps the
+// reference's element order, so the result is the same bit for bit.
+constexpr uint32_t kGatherWindow = 2048;
+
 template <typename VT, int BC>
-__device__ __forceinline__ void gather_body(const uint32_t* __restrict__ off, const uint32_t* __restrict__ pos,
-                                            const VT* __restrict__ ve, const VT* __restrict__ u,
-                                            const uint8_t* __restrict__ flag, const double* __restrict__ coef,
-                                            VT* __restrict__ v, uint32_t d)
+__device__ __forceinline__ void gather_block(const uint32_t* __restrict__ off, const uint32_t* __restrict__ pos,
+                                             const VT* __restrict__ ve, const VT* __restrict__ u,
+                                             const uint8_t* __restrict__ flag, const double* __restrict__ coef,
+                                             VT* __restrict__ v, uint32_t n, uint32_t d0, VT* __restrict__ buf)
 {
-    const uint32_t b = off[d], cnt = off[d + 1] - b;
-    VT s = VT(0);
-    // chunks of 8 entries: all index loads, then all row loads, in flight
-    // together; the sum keeps the list order (entries past the end re-read
-    // the last one and are not added)
-    for (uint32_t q0 = 0; q0 < cnt; q0 += 8) {
-        uint32_t p[8];
-        VT x[8];
+    constexpr uint32_t PER = kGatherWindow / 256u;
+    const uint32_t tid = threadIdx.x, d = d0 + tid;
+    const uint32_t dn = min(256u, n - d0);
+    const uint32_t e0 = off[d0], e1 = off[d0 + dn];  // uniform
+    uint32_t b = 0, c = 0;
+    if (d < n) b = off[d], c = off[d + 1];
+    VT sum = VT(0);
+    for (uint32_t w = e0; w < e1; w += kGatherWindow) {
+        const uint32_t cnt = min(kGatherWindow, e1 - w);
+        uint32_t p[PER];
+        VT x[PER];
+        // entries past the window's end re-read its last entry and are not kept
 #pragma unroll
-        for (uint32_t j = 0; j < 8; j++) p[j] = pos[b + min(q0 + j, cnt - 1)];
+        for (uint32_t j = 0; j < PER; j++) p[j] = pos[w + min(tid + 256u * j, cnt - 1u)];
 #pragma unroll
-        for (uint32_t j = 0; j < 8; j++) x[j] = ve[p[j]];
+        for (uint32_t j = 0; j < PER; j++) x[j] = ve[p[j]];
 #pragma unroll
-        for (uint32_t j = 0; j < 8; j++)
-            if (q0 + j < cnt) s += x[j];
+        for (uint32_t j = 0; j < PER; j++)
+            if (tid + 256u * j < cnt) buf[tid + 256u * j] = x[j];
+        __syncthreads();
+        const uint32_t lo = max(b, w), hi = min(c, w + cnt);
+        for (uint32_t q = lo; q < hi; q++) sum += buf[q - w];
+        __syncthreads();  // the window's sums precede the next window's stores
     }
+    if (d >= n) return;
     if constexpr (BC == 1) {
-        if (flag[d]) s = u[d];
+        if (flag[d]) sum = u[d];
     } else if constexpr (BC == 2) {
-        if (flag[d]) s = (VT)((double)s + coef[d] * (double)u[d]);  // penalty row in FP64
+        if (flag[d]) sum = (VT)((double)sum + coef[d] * (double)u[d]);  // penalty row in FP64
     }
-    v[d] = s;
+    v[d] = sum;
 }
 
 template <typename VT, int BC>
@@ -826,10 +846,11 @@ __global__ __launch_bounds__(256) void gather_kernel(const uint32_t* __restrict_
                                                     const uint8_t* __restrict__ flag, const double* __restrict__ coef,
                                                     VT* __restrict__ v, uint32_t n, const uint32_t* __restrict__ skip)
 {
+    __shared__ VT buf[kGatherWindow];
     if (skip && *skip) return;
-    const uint32_t d = xcd_block(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
-    if (d >= n) return;
-    gather_body<VT, BC>(off, pos, ve, u, flag, coef, v, d);
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x)) * 256u;
+    if (d0 >= n) return;  // uniform per block
+    gather_block<VT, BC>(off, pos, ve, u, flag, coef, v, n, d0, buf);
 }
 
 // many jobs in one launch: block -> 256 DoFs of entry owner_of(prefix)
@@ -837,19 +858,20 @@ template <typename VT>
 __global__ __launch_bounds__(256) void gather_batched_kernel(const GatherJob* __restrict__ jobs,
                                                             const uint32_t* __restrict__ prefix, uint32_t n_jobs)
 {
+    __shared__ VT buf[kGatherWindow];
     const uint32_t gb = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x));
     if (gb >= prefix[n_jobs]) return;
     const uint32_t j = owner_of(prefix, n_jobs, gb);
     const GatherJob& jb = jobs[j];
     if (jb.skip && *jb.skip) return;
-    const uint32_t d = (gb - prefix[j]) * 256u + threadIdx.x;
-    if (d >= jb.n) return;
+    const uint32_t d0 = (gb - prefix[j]) * 256u;
+    if (d0 >= jb.n) return;
     const VT* ve = static_cast<const VT*>(jb.ve);
     const VT* u = static_cast<const VT*>(jb.u);
     VT* v = static_cast<VT*>(jb.v);
-    if (jb.bc == 1) gather_body<VT, 1>(jb.off, jb.pos, ve, u, jb.flag, jb.coef, v, d);
-    else if (jb.bc == 2) gather_body<VT, 2>(jb.off, jb.pos, ve, u, jb.flag, jb.coef, v, d);
-    else gather_body<VT, 0>(jb.off, jb.pos, ve, u, jb.flag, jb.coef, v, d);
+    if (jb.bc == 1) gather_block<VT, 1>(jb.off, jb.pos, ve, u, jb.flag, jb.coef, v, jb.n, d0, buf);
+    else if (jb.bc == 2) gather_block<VT, 2>(jb.off, jb.pos, ve, u, jb.flag, jb.coef, v, jb.n, d0, buf);
+    else gather_block<VT, 0>(jb.off, jb.pos, ve, u, jb.flag, jb.coef, v, jb.n, d0, buf);
 }
 
 template <typename VT, int BC>
