@@ -192,6 +192,46 @@ def test_mixed_element_sizes_and_generic_kernel(torch_gpu):
     np.testing.assert_array_equal(v, vo)
 
 
+def test_long_lists_span_gather_windows(torch_gpu):
+    """The gather reads a block's list range in windows of 2,048 entries: DoFs
+    shared by hundreds to thousands of elements make a block's range several
+    windows long and single lists cross window boundaries; bit-exact against
+    the oracle's sequential matvec_ghosted_OMP (lists summed in element order)."""
+    import ctypes as C
+    rng = np.random.default_rng(11)
+    N, E, n = 1200, 4000, 8
+    hubs = [0, 3, 255, 256, 700]                 # within, at the end of and after the first block
+    maps_l = []
+    for e in range(E):
+        h = 0 if e % 2 == 0 else hubs[(e // 2) % len(hubs)]
+        rest = rng.choice(np.setdiff1d(np.arange(N), [h]), n - 1, replace=False)
+        mp = np.concatenate([[h], rest]).astype(np.uint32)
+        rng.shuffle(mp)
+        maps_l.append(mp)
+    mats = [rng.normal(size=(n, n)) for _ in range(E)]
+    counts = np.bincount(np.concatenate(maps_l), minlength=N)
+    assert counts[:256].sum() > 3 * 2048 and counts[0] > 2048
+    m = Maps()
+    m.set_map(E, maps_l, np.full(E, n, np.uint32), N, np.arange(N), 0, N - 1, N)
+    op = aMatFree(m)
+    op.set_matfree_type(3)
+    for e, A in enumerate(mats):
+        op.set_element_matrix(e, A)
+    op.set_bc_operator(BCOP.NONE)
+    op.finalize()
+    u = mesh.uniform(2, 2, N, -1, 1)
+    v = run(torch_gpu, op.matvec_dev, u)
+    K_flat = np.concatenate([A.reshape(-1) for A in mats])
+    lm = np.concatenate(maps_l).astype(np.uint32)
+    dpe = np.full(E, n, np.uint32)
+    vo = np.zeros(N)
+    order = np.arange(E, dtype=np.uint32)
+    orc.lib().orc_matvec_ghosted_omp(C.c_uint32(E), orc._p(dpe), orc._p(lm), orc._p(K_flat), C.c_int(0),
+                                     orc._p(order), C.c_uint32(E), None, C.c_uint32(0), orc._p(u),
+                                     orc._p(vo), C.c_uint32(N), C.c_int(1))
+    np.testing.assert_array_equal(v, vo)
+
+
 def test_operator_without_elements(torch_gpu):
     """A rank that owns DoFs but no elements: K = 0, so matvec gives 0 and
     MatMult_mf only its identity rows v_c = u_c (include/aMatFree.hpp:3916-3933)."""
