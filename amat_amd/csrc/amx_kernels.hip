@@ -684,6 +684,8 @@ __device__ __forceinline__ uint32_t owner_of(const uint32_t* __restrict__ prefix
 
 // Many operators (jobs) in one launch: wave -> tile of entry owner_of(prefix).
 // Same per-tile arithmetic as ebe_tile_kernel (deterministic scatter only).
+// Jobs with near-symmetric records or packed records with exceptions run in
+// rec_batched_kernel instead.
 template <int N, typename KT, typename VT>
 __global__ __launch_bounds__(256) void ebe_tile_batched_kernel(const EbeJob* __restrict__ jobs,
                                                               const uint32_t* __restrict__ prefix, uint32_t n_jobs)
@@ -696,36 +698,6 @@ __global__ __launch_bounds__(256) void ebe_tile_batched_kernel(const EbeJob* __r
     const EbeJob& jb = jobs[j];
     if (jb.skip && *jb.skip) return;
     const uint32_t t = jb.tile_begin + (gt - prefix[j]);
-    if constexpr (N == 60 && !std::is_same<KT, double>::value) {
-        if (jb.nsym) {  // packed symmetric records with exceptions: one region per wave
-            extern __shared__ __align__(16) char psx_lds[];
-            const uint32_t nrv = psx_record_values(60, jb.nsym_nx, (uint32_t)sizeof(KT)), lane = threadIdx.x & 63u;
-            char* wl = psx_lds + (threadIdx.x >> 6) * psx_region_bytes<VT>(60, psx_max_x<60, KT>(), sizeof(KT));
-            uint32_t m;
-            const KT* rec = static_cast<const KT*>(jb.K) + (uint64_t)t * nrv;
-            const VT acc = jb.masked ? psx_acc<60, KT, VT, true>(rec, nrv, jb.nsym_xs, jb.map,
-                                                                 static_cast<const VT*>(jb.u), jb.kdiag, t, wl, m)
-                                     : psx_acc<60, KT, VT, false>(rec, nrv, jb.nsym_xs, jb.map,
-                                                                  static_cast<const VT*>(jb.u), jb.kdiag, t, wl, m);
-            if (lane < 60u) static_cast<VT*>(jb.out)[jb.ve_base + (uint64_t)lane * jb.ve_rs + (uint64_t)t * jb.ve_ss] = acc;
-            return;
-        }
-    }
-    if constexpr (N == 60 && std::is_same<KT, double>::value && std::is_same<VT, double>::value) {
-        if (jb.nsym) {  // near-symmetric records: one region per wave in the launch's dynamic LDS
-            extern __shared__ __align__(16) double nsym_lds[];
-            const uint32_t nxp = nsym_nxp(jb.nsym_nx), lane = threadIdx.x & 63u, lc = lane < 60u ? lane : 59u;
-            const uint8_t* rec = static_cast<const uint8_t*>(jb.K) + (uint64_t)t * nsym_record_bytes_of(60, jb.nsym_nx);
-            double* wl = nsym_lds + (threadIdx.x >> 6) * nsym_region_values(60, nsym_max_x<60>());
-            uint32_t m;
-            const double acc = jb.masked ? ebe_nsym_acc<60, true>(rec, nxp, jb.nsym_xs[lc], jb.nsym_colany, jb.map,
-                                                                  static_cast<const double*>(jb.u), t, wl, m)
-                                         : ebe_nsym_acc<60, false>(rec, nxp, jb.nsym_xs[lc], jb.nsym_colany, jb.map,
-                                                                   static_cast<const double*>(jb.u), t, wl, m);
-            if (lane < 60u) static_cast<double*>(jb.out)[jb.ve_base + (uint64_t)lane * jb.ve_rs + (uint64_t)t * jb.ve_ss] = acc;
-            return;
-        }
-    }
     if constexpr (SYM_OK) {
         if (jb.sym) {
             KT* reg = ksh + (threadIdx.x >> 6) * sym_region_values<N, KT>();
@@ -747,6 +719,42 @@ __global__ __launch_bounds__(256) void ebe_tile_batched_kernel(const EbeJob* __r
         ebe_tile_body<N, KT, VT, false, false>(static_cast<const KT*>(jb.K), jb.map, static_cast<const VT*>(jb.u),
                                                static_cast<VT*>(jb.out), jb.ve_base, jb.ve_rs, jb.ve_ss, t,
                                                jb.kdiag);
+}
+
+// Many jobs whose 60-DoF groups hold near-symmetric records (FP64) or packed
+// records with exceptions (FP32 / BF16), in one launch with the one-operator
+// kernels' shape: one-wave blocks, block -> element of entry owner_of(prefix),
+// the dynamic LDS one region sized for the largest exception count of the
+// launch's jobs (so as many waves stay resident per CU as in a one-operator
+// launch).  Same arithmetic as ebe_nsym_kernel / psx_kernel.  MASKED: every
+// job of the launch has the identity BC form.
+template <typename KT, typename VT, bool MASKED>
+__global__ __launch_bounds__(64) void rec_batched_kernel(const EbeJob* __restrict__ jobs,
+                                                        const uint32_t* __restrict__ prefix, uint32_t n_jobs)
+{
+    extern __shared__ __align__(16) char rec_lds[];
+    const uint32_t gt = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x));
+    if (gt >= prefix[n_jobs]) return;
+    const uint32_t j = owner_of(prefix, n_jobs, gt);
+    const EbeJob& jb = jobs[j];
+    if (jb.skip && *jb.skip) return;
+    const uint32_t t = jb.tile_begin + (gt - prefix[j]), lane = threadIdx.x;
+    uint32_t m;
+    VT acc;
+    if constexpr (std::is_same<KT, double>::value) {
+        static_assert(std::is_same<VT, double>::value, "near-symmetric records: FP64 vectors");
+        const uint32_t nxp = nsym_nxp(jb.nsym_nx), lc = lane < 60u ? lane : 59u;
+        const uint8_t* rec = static_cast<const uint8_t*>(jb.K) + (uint64_t)t * nsym_record_bytes_of(60, jb.nsym_nx);
+        double* wl = reinterpret_cast<double*>(rec_lds);
+        acc = ebe_nsym_acc<60, MASKED>(rec, nxp, jb.nsym_xs[lc], jb.nsym_colany, jb.map,
+                                       static_cast<const double*>(jb.u), t, wl, m);
+    } else {
+        const uint32_t nrv = psx_record_values(60, jb.nsym_nx, (uint32_t)sizeof(KT));
+        const KT* rec = static_cast<const KT*>(jb.K) + (uint64_t)t * nrv;
+        acc = psx_acc<60, KT, VT, MASKED>(rec, nrv, jb.nsym_xs, jb.map, static_cast<const VT*>(jb.u), jb.kdiag, t,
+                                          rec_lds, m);
+    }
+    if (lane < 60u) static_cast<VT*>(jb.out)[jb.ve_base + (uint64_t)lane * jb.ve_rs + (uint64_t)t * jb.ve_ss] = acc;
 }
 
 // Any n: one wave per element, column-major storage (G = 1, P = 1), element
@@ -1531,32 +1539,34 @@ void launch_gather(bool vt_double, int bc, const uint32_t* off, const uint32_t* 
 }
 
 void launch_ebe_batched(uint32_t n, KType kt, bool vt_double, const EbeJob* jobs, const uint32_t* prefix,
-                        uint32_t n_jobs, uint32_t total_tiles, hipStream_t s, bool any_nsym)
+                        uint32_t n_jobs, uint32_t total_tiles, hipStream_t s, bool records, uint32_t max_nx,
+                        bool masked)
 {
     if (total_tiles == 0 || n_jobs == 0) return;
-    const uint32_t nb = blocks_for(total_tiles, 4);
-    // near-symmetric records: four regions of the largest record a job may have
-    size_t lds = 0;
-    if (any_nsym) {
+    if (records) {  // every job: 60-DoF records, one element per tile
         AMX_REQUIRE(n == 60 && (kt != KType::F64 || vt_double), AMX_INVALID_STATE,
-                    "near-symmetric records hold matrices of 60 DoFs");
-        const void* fn = nullptr;
-        if (kt == KType::F64) {
-            lds = 4u * nsym_region_values(60, nsym_max_x<60>()) * sizeof(double);
-            fn = reinterpret_cast<const void*>(&ebe_tile_batched_kernel<60, double, double>);
-        } else if (kt == KType::F32) {
-            lds = 4u * (vt_double ? psx_region_bytes<double>(60, psx_max_x<60, float>(), 4)
-                                  : psx_region_bytes<float>(60, psx_max_x<60, float>(), 4));
-            fn = vt_double ? reinterpret_cast<const void*>(&ebe_tile_batched_kernel<60, float, double>)
-                           : reinterpret_cast<const void*>(&ebe_tile_batched_kernel<60, float, float>);
-        } else {
-            lds = 4u * (vt_double ? psx_region_bytes<double>(60, psx_max_x<60, uint16_t>(), 2)
-                                  : psx_region_bytes<float>(60, psx_max_x<60, uint16_t>(), 2));
-            fn = vt_double ? reinterpret_cast<const void*>(&ebe_tile_batched_kernel<60, uint16_t, double>)
-                           : reinterpret_cast<const void*>(&ebe_tile_batched_kernel<60, uint16_t, float>);
-        }
-        AMX_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                    "near-symmetric records hold matrices of 60 DoFs with FP64 vectors");
+        AMX_REQUIRE(max_nx <= nsym_max_exceptions(60, kt), AMX_INVALID_STATE, "more exceptions than a record holds");
+        const uint32_t ks = kt == KType::F64 ? 8u : kt == KType::F32 ? 4u : 2u;
+        const size_t lds = kt == KType::F64 ? nsym_region_values(60, max_nx) * sizeof(double)
+                           : vt_double      ? psx_region_bytes<double>(60, max_nx, ks)
+                                            : psx_region_bytes<float>(60, max_nx, ks);
+#define AMX_REC_B(KTT, VTT)                                                                                        \
+    do {                                                                                                           \
+        if (masked) rec_batched_kernel<KTT, VTT, true><<<total_tiles, 64, lds, s>>>(jobs, prefix, n_jobs);         \
+        else rec_batched_kernel<KTT, VTT, false><<<total_tiles, 64, lds, s>>>(jobs, prefix, n_jobs);               \
+    } while (0)
+        if (kt == KType::F64) AMX_REC_B(double, double);
+        else if (kt == KType::F32 && vt_double) AMX_REC_B(float, double);
+        else if (kt == KType::F32) AMX_REC_B(float, float);
+        else if (vt_double) AMX_REC_B(uint16_t, double);
+        else AMX_REC_B(uint16_t, float);
+#undef AMX_REC_B
+        AMX_HIP(hipGetLastError());
+        return;
     }
+    const uint32_t nb = blocks_for(total_tiles, 4);
+    const size_t lds = 0;
 #define AMX_EBE_B(NN, KTT, VTT) ebe_tile_batched_kernel<NN, KTT, VTT><<<nb, 256, lds, s>>>(jobs, prefix, n_jobs)
 #define AMX_EBE_N(KTT, VTT)                                                                                        \
     switch (n) {                                                                                                   \

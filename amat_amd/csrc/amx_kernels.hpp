@@ -140,10 +140,13 @@ struct GatherJob {
     const uint32_t* skip;
 };
 bool batched_n(uint32_t n);
-// any_nsym: some job has near-symmetric records (the launch then carries
-// their LDS regions)
+// records: every job's group holds near-symmetric records (FP64) or packed
+// records with exceptions (FP32 / BF16) of 60-DoF elements, at most max_nx
+// exception values each, and every job's masked flag equals `masked` (no job
+// of a launch without `records` holds records)
 void launch_ebe_batched(uint32_t n, KType kt, bool vt_double, const EbeJob* jobs, const uint32_t* prefix,
-                        uint32_t n_jobs, uint32_t total_tiles, hipStream_t s, bool any_nsym = false);
+                        uint32_t n_jobs, uint32_t total_tiles, hipStream_t s, bool records, uint32_t max_nx,
+                        bool masked);
 void launch_gather_batched(bool vt_double, const GatherJob* jobs, const uint32_t* prefix, uint32_t n_jobs,
                            uint32_t total_blocks, hipStream_t s);
 

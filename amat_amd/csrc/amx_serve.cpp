@@ -74,7 +74,7 @@ struct Job {
 struct Table {
     DevBuf dev_jobs, dev_prefix;
     uint32_t n = 0, total = 0;
-    bool any_nsym = false;  // element tables: some job has near-symmetric records
+    uint32_t max_nx = 0;    // element tables of records: the largest exception count
 };
 
 }  // namespace
@@ -92,7 +92,9 @@ struct amx_server {
     uint64_t workspace_bytes = 0;
     uint64_t steps = 0, iterations = 0, completed = 0;
     bool tables_dirty = true;
-    std::map<uint64_t, Table> ebe;  // batched element launches, key: n | kt << 16 | f64 << 24
+    // batched element launches, key: n | kt << 16 | f64 << 24 | records << 25 |
+    // (records) identity BC form << 26
+    std::map<uint64_t, Table> ebe;
     Table gather[2], cg[2];         // [f64]
     std::vector<Job*> free_jobs;    // jobs on method-2 (FREE) operators: element kernels launched per job
     // the per-job FREE element kernels run on kAux side streams (default
@@ -168,7 +170,10 @@ void rebuild_tables(amx_server& S)
         if (op.is_free()) S.free_jobs.push_back(&j);
         for (auto& grp : op.groups) {
             if (op.is_free()) break;
-            const uint64_t key = (uint64_t)grp.n | ((uint64_t)j.kt << 16) | ((uint64_t)f << 24);
+            // 60-DoF records: one-wave-block launches, one per BC form
+            const bool rec = grp.L[(int)j.kt].nsym, masked = op.bc_operator == AMX_BCOP_IDENTITY;
+            const uint64_t key = (uint64_t)grp.n | ((uint64_t)j.kt << 16) | ((uint64_t)f << 24) |
+                                 ((uint64_t)(rec ? 1 : 0) << 25) | ((uint64_t)(rec && masked ? 1 : 0) << 26);
             auto& e = ebe[key];
             if (e.second.empty()) e.second.push_back(0);
             amx::EbeJob a{};
@@ -225,8 +230,8 @@ void rebuild_tables(amx_server& S)
     for (auto& kv : ebe) {
         Table& t = S.ebe[kv.first];
         upload_table(t, kv.second.first, kv.second.second);
-        t.any_nsym = std::any_of(kv.second.first.begin(), kv.second.first.end(),
-                                 [](const amx::EbeJob& a) { return a.nsym != 0; });
+        t.max_nx = 0;
+        for (const auto& a : kv.second.first) t.max_nx = std::max(t.max_nx, a.nsym_nx);
     }
     for (int f = 0; f < 2; f++) {
         upload_table(S.gather[f], g[f], gp[f]);
@@ -243,10 +248,11 @@ void launch_iteration(amx_server& S)
     for (auto& kv : S.ebe) {
         const uint32_t n = (uint32_t)(kv.first & 0xffff);
         const KType kt = (KType)((kv.first >> 16) & 0xff);
-        const bool f64 = ((kv.first >> 24) & 1) != 0;
+        const bool f64 = ((kv.first >> 24) & 1) != 0, rec = ((kv.first >> 25) & 1) != 0;
+        const bool masked = ((kv.first >> 26) & 1) != 0;
         Table& t = kv.second;
         amx::launch_ebe_batched(n, kt, f64, t.dev_jobs.as<amx::EbeJob>(), t.dev_prefix.as<uint32_t>(), t.n, t.total,
-                                s, t.any_nsym);
+                                s, rec, t.max_nx, masked);
     }
     if (!S.free_jobs.empty()) {  // element kernels from coordinates (FP64 or FP32) and the job's gather
         const int na = std::min<int>(amx_server::kAux, (int)S.free_jobs.size());
