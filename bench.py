@@ -437,11 +437,13 @@ def batch_mode(args) -> dict:
 
 
 def extras(args) -> dict:
-    """The precision and batching workloads measured in the default run, in
-    brief: jittered tet10 Poisson 96^3 x 6 (matvec per storage, FP64 CG
-    against FP32 / BF16 storage with FP64 reliable updates, to 1e-10), and the
+    """The precision, batching and serving workloads measured in the default
+    run, in brief: jittered tet10 Poisson 96^3 x 6 (matvec per storage, FP64 CG
+    against FP32 / BF16 storage with FP64 reliable updates, to 1e-10), the
     multi-RHS products of hex20 48^3 + hex8 80x40x20 (best autotuned launch
-    per storage and nrhs over both scatter forms)."""
+    per storage and nrhs over both scatter forms), and a served stream of 400
+    CG solve requests at 800 req/s (FP64 CG and FP32 storage with reliable
+    updates on stored operators, FP32 inner solves on method-2 operators)."""
     import copy
     import torch
     out = {}
@@ -475,6 +477,16 @@ def extras(args) -> dict:
             best[key] = {"ms": ms, "scatter": c["scatter"], "tile": c["tile"], "hbm_frac": c["hbm_frac"],
                          "tflops": c["tflops"], "col0_rel_err_vs_fp64": c["col0_rel_err_vs_fp64"]}
     out["batch_hex20_48_hex8_80x40x20"] = {"best": best, "seconds_in_bench": time.time() - t0}
+    t0 = time.time()
+    c = copy.copy(args)
+    c.serve_requests, c.serve_rates, c.serve_methods = 400, "800", "fp64,fp32r,fp32rfree"
+    sr = serve_mode(c)
+    torch.cuda.empty_cache()
+    keep = ("solves_per_s", "p50_s", "p99_s", "service_p50_s", "service_p99_s", "iterations", "all_converged",
+            "max_rel_residual", "speedup_same_form", "speedup_vs_stored_fp64")
+    out["serve_400_at_800"] = {"requests": sr["requests"], "operators": sr["operators"],
+                               **{r["method"]: {k: r[k] for k in keep if k in r} for r in sr["runs"]},
+                               "seconds_in_bench": time.time() - t0}
     return out
 
 
