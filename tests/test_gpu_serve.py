@@ -185,3 +185,39 @@ def test_serving_loop_small_stream(torch_gpu):
             c = cache.get((d.kind, d.n))
             xa, _ = solve_alone(c.op, c.problem, rhs_for(c.problem, reqs[d.rid].seed), method)
             np.testing.assert_array_equal(d.x, xa)
+
+
+def test_penalty_bc_records_served(torch_gpu):
+    """hex20 elasticity with near-symmetric FP64 records and packed FP32 / BF16
+    records under the penalty BC form (the record launch without the identity
+    mask), beside an identity-BC job: served solves equal the single solves bit
+    for bit."""
+    from amat_amd import BCOP, Maps, aMatFree, problems
+    torch = torch_gpu
+    p = problems.elasticity_hex20(3, 3, 4)
+    op = p.build_operator(aMatFree, Maps, precision=PREC.FP64 | PREC.FP32 | PREC.BF16, bcop=BCOP.PENALTY_DIAG,
+                          penalty=1e8)
+    assert op.info().near_symmetric == 1
+    b = rhs_for(p, 17)
+    other_p = build_problem("hex20", 3)
+    other_op, other_b = build_operator(other_p), rhs_for(other_p, 18)
+    # BF16 refinement under a 1e8 penalty: three outer steps of at most 400
+    # iterations each (not converged), served and single alike
+    for method, max_outer, max_it in ((PREC.FP64, 60, 20000), (PREC.FP32, 60, 20000), (FP32R, 60, 20000),
+                                      (PREC.BF16, 3, 400)):
+        db = torch.from_numpy(b).cuda()
+        x = torch.zeros_like(db)
+        if method == PREC.FP64:
+            st = op.cg(db, x, rtol=1e-10, max_it=max_it)
+        else:
+            st = op.ir_cg(method, db, x, rtol=1e-10, inner_rtol=1e-2, max_outer=max_outer, max_inner=max_it)
+        assert st.converged or method == PREC.BF16
+        s = Server(max_jobs=4)
+        j1 = s.submit(op, method, b, max_it=max_it, max_outer=max_outer)
+        j2 = s.submit(other_op, method, other_b, max_it=max_it, max_outer=max_outer)
+        res = run_all(s, [j1, j2], {j1: p.n_dofs, j2: other_p.n_dofs})
+        xs, ss = res[j1]
+        np.testing.assert_array_equal(xs, x.cpu().numpy())
+        assert ss.iterations == st.iterations and ss.outer_iterations == st.outer_iterations
+        if method != PREC.BF16:
+            np.testing.assert_array_equal(res[j2][0], solve_alone(other_op, other_p, other_b, method)[0])
