@@ -6,11 +6,19 @@
 //                    read), streams its row of K_e as P-column packs with
 //                    non-temporal 16-byte loads, and forms ve = K_e u_e with
 //                    the column values broadcast across the wave (readlane for
-//                    G = 1, ds_bpermute otherwise).  K has no reuse, so it is
-//                    never staged in LDS.
+//                    G = 1, ds_bpermute otherwise); whole matrices are never
+//                    staged in LDS, packed upper triangles of multi-element
+//                    tiles are (each lane then reads its row from the wave's copy).
+//   ebe_nsym_kernel  60-DoF elements held as near-symmetric FP64 records
+//   psx_kernel       (FP32 / BF16: packed records with exceptions): one wave per
+//                    element, the record staged in the wave's LDS region, every
+//                    entry rebuilt bit for bit (amx_nsym.hpp).
 //   gather_kernel    v[d] = sum of the element rows that touch d, in the
 //                    reference's element order (node -> (element, row)
-//                    transposed lists), with the BC rows fused.
+//                    transposed lists), a block's list range staged in LDS,
+//                    with the BC rows fused.
+// The *_batched kernels run the same arithmetic for many operators (the jobs
+// of the solve server) in one launch.
 // Blocks are remapped so that each of the 8 XCDs walks a contiguous range of
 // tiles / DoFs: its L2 then holds one slab of u / ve instead of all of it.
 #include <hip/hip_runtime.h>
