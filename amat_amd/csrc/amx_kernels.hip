@@ -814,7 +814,11 @@ __global__ __launch_bounds__(256) void ebe_generic_kernel(const KT* __restrict__
 // part of its own DoF's list inside the window, in list order, to its running
 // sum.  The loads no longer follow one list per lane (no dependent chains, no
 // lanes idle while a neighbour walks a long list), and every sum keeps the
-// reference's element order, so the result is the same bit for bit.
+// reference's element order, so the result is the same bit for bit.  The list
+// indices are read once per application and loaded non-temporally, so they
+// do not displace the element rows, map and vectors that the next element
+// kernel and gather find in the caches (hex8 128^3: element kernel 0.143 ->
+// 0.135 ms, gather 49 -> 46 us).
 constexpr uint32_t kGatherWindow = 2048;
 
 template <typename VT, int BC>
@@ -836,7 +840,7 @@ __device__ __forceinline__ void gather_block(const uint32_t* __restrict__ off, c
         VT x[PER];
         // entries past the window's end re-read its last entry and are not kept
 #pragma unroll
-        for (uint32_t j = 0; j < PER; j++) p[j] = pos[w + min(tid + 256u * j, cnt - 1u)];
+        for (uint32_t j = 0; j < PER; j++) p[j] = __builtin_nontemporal_load(pos + w + min(tid + 256u * j, cnt - 1u));
 #pragma unroll
         for (uint32_t j = 0; j < PER; j++) x[j] = ve[p[j]];
 #pragma unroll
