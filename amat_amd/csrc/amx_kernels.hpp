@@ -124,9 +124,9 @@ struct EbeJob {
     const uint32_t* skip;   // the job's converged flag
     const float* kdiag;
     uint32_t masked, sym;   // sym: packed symmetric tiles
-    const uint32_t* nsym_xs;  // near-symmetric records (EbeArgs), nsym_nx exceptions
+    const uint32_t* nsym_xs;  // 60-DoF records (EbeArgs), nsym_nx exceptions
     uint64_t nsym_colany;
-    uint32_t nsym, nsym_nx;
+    uint32_t nsym_nx;
 };
 struct GatherJob {
     const uint32_t* off;

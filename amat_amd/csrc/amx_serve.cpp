@@ -190,7 +190,6 @@ void rebuild_tables(amx_server& S)
             a.kdiag = j.kt == KType::BF16 ? grp.kdiag16.as<float>() : nullptr;
             a.masked = op.bc_operator == AMX_BCOP_IDENTITY ? 1u : 0u;
             a.sym = grp.L[(int)j.kt].sym ? 1u : 0u;
-            a.nsym = grp.L[(int)j.kt].nsym ? 1u : 0u;
             a.nsym_nx = grp.L[(int)j.kt].nx;
             a.nsym_xs = j.kt == KType::F64 ? grp.nsym_xs.as<uint32_t>() : grp.psx_tab[(int)j.kt].as<uint32_t>();
             a.nsym_colany = grp.nsym_colany;
